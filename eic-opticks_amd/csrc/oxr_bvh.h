@@ -1,0 +1,36 @@
+// oxr_bvh.h -- host-side binned-SAH BVH builder producing the 64-byte two-child node
+// layout traversed by oxr_trace.h. Replaces the OptiX GAS/IAS builds
+// (CSGOptiX/GAS_Builder.cc:112-159, CSGOptiX/SBT.cc:393-553).
+#pragma once
+#include <vector>
+#include <cstdint>
+
+namespace oxh {
+
+struct Box {
+    double lo[3], hi[3];
+    static Box empty();
+    void grow(const Box& b);
+    void grow_point(const double* p);
+    double area() const;
+    double extent(int axis) const { return hi[axis] - lo[axis]; }
+    double centroid(int axis) const { return 0.5 * (lo[axis] + hi[axis]); }
+};
+
+// Mirrors oxd::BNode: child0 box, child1 box, refs (>=0 node, <0 ~item).
+struct NodeOut {
+    float a[4], b[4], c[4];
+    int32_t ch[4];
+};
+
+struct Bvh {
+    std::vector<NodeOut> nodes;   // BFS order, root at 0 (empty if the root is a leaf)
+    int32_t root_ref = 0;         // 0 for an internal root; ~item if the tree is one leaf
+    int depth = 0;                // internal nodes on the longest root-to-leaf path
+};
+
+// Build over items with the given boxes. node_base is added to every internal ref so
+// several trees can share one concatenated array.
+Bvh build_bvh(const std::vector<Box>& boxes, int node_base);
+
+} // namespace oxh

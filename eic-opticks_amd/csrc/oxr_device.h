@@ -1,0 +1,643 @@
+// oxr_device.h -- device-side vector math, Philox RNG, tables and CSG intersection.
+//
+// Float semantics are deliberate: sources are compiled with -ffp-contract=off and
+// correctly rounded fp32 divide/sqrt, and log/sin/cos/atan2 are evaluated in fp64 and
+// rounded, so every fp32 result is a function of its inputs alone.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OXD __device__ __forceinline__
+
+namespace oxd {
+
+// ------------------------------------------------------------------ vectors
+struct f3 { float x, y, z; };
+OXD f3 mk(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+OXD f3 add(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+OXD f3 sub(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+OXD f3 neg(f3 a) { return mk(-a.x, -a.y, -a.z); }
+OXD f3 mul(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+OXD f3 mulr(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+OXD f3 divs(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+OXD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+OXD f3 cross(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+OXD float len(f3 a) { return sqrtf(dot(a, a)); }
+OXD f3 normalize(f3 a) { float inv = 1.0f / sqrtf(dot(a, a)); return mulr(a, inv); }
+
+OXD float xlog(float x) { return (float)log((double)x); }
+OXD float xsin(float x) { return (float)sin((double)x); }
+OXD float xcos(float x) { return (float)cos((double)x); }
+OXD float xatan2(float y, float x) { return (float)atan2((double)y, (double)x); }
+constexpr float PIf = 3.14159265358979323846f;
+
+// ------------------------------------------------------------------ flags
+enum : uint32_t {
+    CERENKOV = 1u << 0, SCINTILLATION = 1u << 1, TORCH = 1u << 2, BULK_ABSORB = 1u << 3,
+    BULK_REEMIT = 1u << 4, BULK_SCATTER = 1u << 5, SURFACE_DETECT = 1u << 6,
+    SURFACE_ABSORB = 1u << 7, SURFACE_DREFLECT = 1u << 8, SURFACE_SREFLECT = 1u << 9,
+    BOUNDARY_REFLECT = 1u << 10, BOUNDARY_TRANSMIT = 1u << 11
+};
+enum { BREAK = 1, CONTINUE = 2, BOUNDARY = 3 };
+enum { CSG_ZERO = 0, CSG_NODE = 11, CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103,
+       CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110, CSG_CONVEXPOLYHEDRON = 112,
+       CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
+
+// ------------------------------------------------------------------ Philox4x32-10
+// Counter-based generator of Salmon et al. (SC'11). Stream of photon p: key = seed,
+// counter = (block lo, block hi, p lo, p hi), uniform j at block (start+j)/4, lane (start+j)%4.
+struct Rng {
+    uint32_t k0, k1;
+    uint32_t p0, p1;          // photon index (subsequence)
+    uint64_t pos;             // absolute uniform index
+    uint32_t b[4];
+    uint64_t blk;
+    const float* pre;         // precooked row or null
+    int32_t npre, cur;
+    uint32_t overflow;
+};
+
+OXD void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t* o) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+}
+
+OXD void rng_init(Rng& r, uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event, uint64_t photon) {
+    r.k0 = uint32_t(seed); r.k1 = uint32_t(seed >> 32);
+    r.p0 = uint32_t(photon); r.p1 = uint32_t(photon >> 32);
+    r.pos = offset + skip * event;
+    r.blk = ~0ull;
+    r.pre = nullptr; r.npre = 0; r.cur = 0; r.overflow = 0;
+}
+
+OXD float uniform(Rng& r) {
+    if (r.pre) {
+        if (r.cur >= r.npre) { r.cur = 0; r.overflow++; }
+        return r.pre[r.cur++];
+    }
+    uint64_t block = r.pos >> 2;
+    if (block != r.blk) {
+        philox(uint32_t(block), uint32_t(block >> 32), r.p0, r.p1, r.k0, r.k1, r.b);
+        r.blk = block;
+    }
+    uint32_t j = uint32_t(r.pos & 3);
+    uint32_t x = j == 0 ? r.b[0] : (j == 1 ? r.b[1] : (j == 2 ? r.b[2] : r.b[3]));
+    r.pos++;
+    return float((x >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// ------------------------------------------------------------------ tables
+// bnd texture emulation: linear filter, normalised coordinates, wrap (qudarap/qbnd.h:103-125)
+struct Tables {
+    const float4* bnd;      // rows of num_wl float4, row = 2*line + k
+    const uint4* optical;   // (num_bnd*4)
+    const float* icdf;      // (rows, width)
+    int num_wl;
+    float wl_min, wl_step;
+    int icdf_width, hd_factor;
+};
+
+OXD float4 boundary_lookup(const Tables& t, float nm, unsigned line, unsigned k) {
+    const int nx = t.num_wl;
+    float fx = (nm - t.wl_min) / t.wl_step;
+    float x = (fx + 0.5f) / float(nx);
+    unsigned iy = 2u * line + k;
+    float xf = x * float(nx) - 0.5f;
+    float fl = floorf(xf);
+    float a = xf - fl;
+    int i0 = int(fl);
+    int m0 = i0 % nx; if (m0 < 0) m0 += nx;
+    int m1 = m0 + 1; if (m1 == nx) m1 = 0;
+    const float4* row = t.bnd + size_t(iy) * nx;
+    float4 v0 = row[m0], v1 = row[m1];
+    float b = 1.0f - a;
+    return make_float4(b * v0.x + a * v1.x, b * v0.y + a * v1.y, b * v0.z + a * v1.z, b * v0.w + a * v1.w);
+}
+
+OXD float icdf_lookup(const Tables& t, float x, int row) {
+    const int nx = t.icdf_width;
+    float xf = x * float(nx) - 0.5f;
+    float fl = floorf(xf);
+    float a = xf - fl;
+    int i0 = int(fl);
+    int m0 = i0 % nx; if (m0 < 0) m0 += nx;
+    int m1 = m0 + 1; if (m1 == nx) m1 = 0;
+    const float* r = t.icdf + size_t(row) * nx;
+    return (1.0f - a) * r[m0] + a * r[m1];
+}
+
+// qscint::wavelength_hd0/10/20 (qudarap/qscint.h:150-228)
+OXD float scint_wavelength(const Tables& t, float u0) {
+    if (!t.icdf) return 0.f;
+    if (t.hd_factor == 20) {
+        if (u0 < 0.05f) return icdf_lookup(t, u0 * 20.f, 1);
+        if (u0 > 0.95f) return icdf_lookup(t, (u0 - 0.95f) * 20.f, 2);
+        return icdf_lookup(t, u0, 0);
+    }
+    if (t.hd_factor == 10) {
+        if (u0 < 0.1f) return icdf_lookup(t, u0 * 10.f, 1);
+        if (u0 > 0.9f) return icdf_lookup(t, (u0 - 0.9f) * 10.f, 2);
+        return icdf_lookup(t, u0, 0);
+    }
+    return icdf_lookup(t, u0, 0);
+}
+
+// ------------------------------------------------------------------ CSG
+struct Is { float x, y, z, w; };
+
+// CSGNode as four float4: q0 params, q1 (z1, z2, boundary, index), q2/q3 aabb, q3.z typecode,
+// q3.w gtransformIdx|complement (CSG/CSGNode.h:67-98)
+struct NodeV { float4 q0, q1, q3; };
+OXD NodeV load_node(const float4* node, int i) {
+    NodeV n;
+    n.q0 = node[4 * i + 0];
+    n.q1 = node[4 * i + 1];
+    n.q3 = node[4 * i + 3];
+    return n;
+}
+OXD unsigned typecode(const NodeV& n) { return __float_as_uint(n.q3.z); }
+
+// csg_robust_quadratic_roots.h:95-134
+OXD void rq_roots(float& t1, float& t2, float& sdisc, float d, float b, float c) {
+    float disc = b * b - d * c;
+    sdisc = disc > 0.f ? sqrtf(disc) : 0.f;
+    float q = b > 0.f ? -(b + sdisc) : -(b - sdisc);
+    float r1 = q / d, r2 = c / q;
+    t1 = fminf(r1, r2); t2 = fmaxf(r1, r2);
+}
+OXD void rq_roots64(double& t1, double& t2, double& sdisc, double d, double b, double c) {
+    double disc = b * b - d * c;
+    sdisc = disc > 0.0 ? sqrt(disc) : 0.0;
+    double q = b > 0.0 ? -(b + sdisc) : -(b - sdisc);
+    double r1 = q / d, r2 = c / q;
+    t1 = fmin(r1, r2); t2 = fmax(r1, r2);
+}
+OXD void rq_roots_disq(float t_min, float& t1, float& t2, float& sdisc, float d, float b, float c) {
+    float disc = b * b - d * c;
+    sdisc = disc > 0.f ? sqrtf(disc) : 0.f;
+    float q = b > 0.f ? -(b + sdisc) : -(b - sdisc);
+    float r1 = sdisc > 0.f ? q / d : t_min;
+    float r2 = sdisc > 0.f ? c / q : t_min;
+    t1 = fminf(r1, r2); t2 = fmaxf(r1, r2);
+}
+OXD void rq_roots_disq64(double t_min, double& t1, double& t2, double& sdisc, double d, double b, double c) {
+    double disc = b * b - d * c;
+    sdisc = disc > 0.0 ? sqrt(disc) : 0.0;
+    double q = b > 0.0 ? -(b + sdisc) : -(b - sdisc);
+    double r1 = sdisc > 0.0 ? q / d : t_min;
+    double r2 = sdisc > 0.0 ? c / q : t_min;
+    t1 = fmin(r1, r2); t2 = fmax(r1, r2);
+}
+
+// csg_intersect_leaf_sphere.h:15-55
+template <bool FP64>
+OXD bool leaf_sphere(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
+    f3 O = sub(o, mk(n.q0.x, n.q0.y, n.q0.z));
+    float radius = n.q0.w;
+    float root1, root2, sdisc;
+    if (FP64) {
+        double ox = O.x, oy = O.y, oz = O.z, dx = d.x, dy = d.y, dz = d.z;
+        double b = ox * dx + oy * dy + oz * dz;
+        double c = (ox * ox + oy * oy + oz * oz) - double(radius) * double(radius);
+        double dd = dx * dx + dy * dy + dz * dz;
+        double r1, r2, sd;
+        rq_roots64(r1, r2, sd, dd, b, c);
+        root1 = float(r1); root2 = float(r2); sdisc = float(sd);
+    } else {
+        float b = dot(O, d);
+        float c = dot(O, O) - radius * radius;
+        float dd = dot(d, d);
+        rq_roots(root1, root2, sdisc, dd, b, c);
+    }
+    float t = sdisc > 0.f ? (root1 > t_min ? root1 : root2) : t_min;
+    bool valid = t > t_min;
+    if (valid) {
+        is.x = (O.x + t * d.x) / radius;
+        is.y = (O.y + t * d.y) / radius;
+        is.z = (O.z + t * d.z) / radius;
+        is.w = t;
+    }
+    return valid;
+}
+
+// csg_intersect_leaf_zsphere.h:45-151
+template <bool FP64>
+OXD bool leaf_zsphere(Is& is, const NodeV& n, float t_min, f3 ro, f3 rd) {
+    f3 center = mk(n.q0.x, n.q0.y, n.q0.z);
+    f3 O = sub(ro, center);
+    f3 D = rd;
+    float radius = n.q0.w;
+    float b = dot(O, D);
+    float c = dot(O, O) - radius * radius;
+    if (c > 0.f && b > 0.f) return false;
+    float zmax = center.z + n.q1.y;
+    float zmin = center.z + n.q1.x;
+    float d = dot(D, D);
+    float t1sph, t2sph, sdisc;
+    if (FP64) {
+        double ox = O.x, oy = O.y, oz = O.z, dx = D.x, dy = D.y, dz = D.z;
+        double bd = ox * dx + oy * dy + oz * dz;
+        double cd = (ox * ox + oy * oy + oz * oz) - double(radius) * double(radius);
+        double dd = dx * dx + dy * dy + dz * dz;
+        double r1, r2, sd;
+        rq_roots64(r1, r2, sd, dd, bd, cd);
+        t1sph = float(r1); t2sph = float(r2); sdisc = float(sd);
+    } else {
+        rq_roots(t1sph, t2sph, sdisc, d, b, c);
+    }
+    float z1sph = ro.z + t1sph * rd.z;
+    float z2sph = ro.z + t2sph * rd.z;
+    float idz = 1.f / rd.z;
+    float t_QCAP = (zmax - ro.z) * idz;
+    float t_PCAP = (zmin - ro.z) * idz;
+    float t1cap = fminf(t_QCAP, t_PCAP);
+    float t2cap = fmaxf(t_QCAP, t_PCAP);
+    if (t1cap < t1sph || t1cap > t2sph) t1cap = t_min;
+    if (t2cap < t1sph || t2cap > t2sph) t2cap = t_min;
+    float t = t_min;
+    if (sdisc > 0.f) {
+        if (t1sph > t_min && z1sph > zmin && z1sph <= zmax) t = t1sph;
+        else if (t1cap > t_min) t = t1cap;
+        else if (t2cap > t_min) t = t2cap;
+        else if (t2sph > t_min && z2sph > zmin && z2sph <= zmax) t = t2sph;
+    }
+    bool valid = t > t_min;
+    if (valid) {
+        is.w = t;
+        if (t == t1sph || t == t2sph) {
+            is.x = (O.x + t * D.x) / radius;
+            is.y = (O.y + t * D.y) / radius;
+            is.z = (O.z + t * D.z) / radius;
+        } else {
+            is.x = 0.f; is.y = 0.f;
+            is.z = t == t_PCAP ? -1.f : 1.f;
+        }
+    }
+    return valid;
+}
+
+// csg_intersect_leaf_cylinder.h:34-82
+template <bool FP64>
+OXD bool leaf_cylinder(Is& is, const NodeV& n, float t_min, f3 o, f3 v) {
+    float r = n.q0.w, z1 = n.q1.x, z2 = n.q1.y;
+    float ox = o.x, oy = o.y, oz = o.z, vx = v.x, vy = v.y, vz = v.z;
+    float r2 = r * r;
+    float t_near, t_far, sdisc;
+    if (FP64) {
+        double ad = double(vx) * vx + double(vy) * vy, bd = double(ox) * vx + double(oy) * vy;
+        double cd = double(ox) * ox + double(oy) * oy - double(r) * double(r);
+        double tn, tf, sd;
+        rq_roots_disq64(double(t_min), tn, tf, sd, ad, bd, cd);
+        t_near = float(tn); t_far = float(tf); sdisc = float(sd);
+    } else {
+        float a = vx * vx + vy * vy;
+        float b = ox * vx + oy * vy;
+        float c = ox * ox + oy * oy - r2;
+        rq_roots_disq(t_min, t_near, t_far, sdisc, a, b, c);
+    }
+    float z_near = oz + t_near * vz;
+    float z_far = oz + t_far * vz;
+    float t_z1cap = (z1 - oz) / vz;
+    float r2_z1cap = (ox + t_z1cap * vx) * (ox + t_z1cap * vx) + (oy + t_z1cap * vy) * (oy + t_z1cap * vy);
+    float t_z2cap = (z2 - oz) / vz;
+    float r2_z2cap = (ox + t_z2cap * vx) * (ox + t_z2cap * vx) + (oy + t_z2cap * vy) * (oy + t_z2cap * vy);
+    const float INF = __int_as_float(0x7f800000);
+    float t = INF;
+    if (t_near > t_min && z_near > z1 && z_near < z2 && t_near < t) t = t_near;
+    if (t_far > t_min && z_far > z1 && z_far < z2 && t_far < t) t = t_far;
+    if (t_z1cap > t_min && r2_z1cap <= r2 && t_z1cap < t) t = t_z1cap;
+    if (t_z2cap > t_min && r2_z2cap <= r2 && t_z2cap < t) t = t_z2cap;
+    bool valid = t > t_min && t < INF;
+    if (valid) {
+        bool sheet = (t == t_near || t == t_far);
+        is.x = sheet ? (ox + t * vx) / r : 0.f;
+        is.y = sheet ? (oy + t * vy) / r : 0.f;
+        is.z = sheet ? 0.f : (t == t_z1cap ? -1.f : 1.f);
+        is.w = t;
+    }
+    return valid;
+}
+
+// csg_intersect_leaf_box3.h:64-157
+OXD bool leaf_box3(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
+    f3 bmin = mk(-n.q0.x / 2.f, -n.q0.y / 2.f, -n.q0.z / 2.f);
+    f3 bmax = mk(n.q0.x / 2.f, n.q0.y / 2.f, n.q0.z / 2.f);
+    f3 idir = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    float t0x = (bmin.x - o.x) * idir.x, t0y = (bmin.y - o.y) * idir.y, t0z = (bmin.z - o.z) * idir.z;
+    float t1x = (bmax.x - o.x) * idir.x, t1y = (bmax.y - o.y) * idir.y, t1z = (bmax.z - o.z) * idir.z;
+    float nx = fminf(t0x, t1x), ny = fminf(t0y, t1y), nz = fminf(t0z, t1z);
+    float fx = fmaxf(t0x, t1x), fy = fmaxf(t0y, t1y), fz = fmaxf(t0z, t1z);
+    float t_near = fmaxf(fmaxf(nx, ny), nz);
+    float t_far = fminf(fminf(fx, fy), fz);
+    bool along_x = d.x != 0.f && d.y == 0.f && d.z == 0.f;
+    bool along_y = d.x == 0.f && d.y != 0.f && d.z == 0.f;
+    bool along_z = d.x == 0.f && d.y == 0.f && d.z != 0.f;
+    bool in_x = o.x > bmin.x && o.x < bmax.x;
+    bool in_y = o.y > bmin.y && o.y < bmax.y;
+    bool in_z = o.z > bmin.z && o.z < bmax.z;
+    bool has;
+    if (along_x) has = in_y && in_z;
+    else if (along_y) has = in_x && in_z;
+    else if (along_z) has = in_x && in_y;
+    else has = (t_far > t_near && t_far > 0.f);
+    if (!has) return false;
+    float t = t_min < t_near ? t_near : (t_min < t_far ? t_far : t_min);
+    f3 p = mk(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
+    float pax = fabsf(p.x) / (bmax.x - bmin.x);
+    float pay = fabsf(p.y) / (bmax.y - bmin.y);
+    float paz = fabsf(p.z) / (bmax.z - bmin.z);
+    f3 nn = mk(0.f, 0.f, 0.f);
+    if (pax >= pay && pax >= paz) nn.x = copysignf(1.f, p.x);
+    else if (pay >= pax && pay >= paz) nn.y = copysignf(1.f, p.y);
+    else if (paz >= pax && paz >= pay) nn.z = copysignf(1.f, p.z);
+    if (t > t_min) {
+        is.x = nn.x; is.y = nn.y; is.z = nn.z; is.w = t;
+        return true;
+    }
+    return false;
+}
+
+// csg_intersect_leaf_newcone.h:50-121
+template <bool FP64>
+OXD bool leaf_cone(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
+    float r1 = n.q0.x, z1 = n.q0.y, r2 = n.q0.z, z2 = n.q0.w;
+    const float RT_MAX = 1e16f;
+    float r1r1 = r1 * r1, r2r2 = r2 * r2;
+    float tth = (r2 - r1) / (z2 - z1);
+    float tth2 = tth * tth;
+    float z0 = (z2 * r1 - z1 * r2) / (r1 - r2);
+    float idz = 1.f / d.z;
+    float t_cap1 = d.z == 0.f ? RT_MAX : (z1 - o.z) * idz;
+    float t_cap2 = d.z == 0.f ? RT_MAX : (z2 - o.z) * idz;
+    float rr_cap1 = (o.x + t_cap1 * d.x) * (o.x + t_cap1 * d.x) + (o.y + t_cap1 * d.y) * (o.y + t_cap1 * d.y);
+    float rr_cap2 = (o.x + t_cap2 * d.x) * (o.x + t_cap2 * d.x) + (o.y + t_cap2 * d.y) * (o.y + t_cap2 * d.y);
+    t_cap1 = rr_cap1 < r1r1 && t_cap1 > t_min ? t_cap1 : RT_MAX;
+    t_cap2 = rr_cap2 < r2r2 && t_cap2 > t_min ? t_cap2 : RT_MAX;
+    float c2 = d.x * d.x + d.y * d.y - d.z * d.z * tth2;
+    float c1 = o.x * d.x + o.y * d.y - (o.z - z0) * d.z * tth2;
+    float c0 = o.x * o.x + o.y * o.y - (o.z - z0) * (o.z - z0) * tth2;
+    float t_near, t_far, sdisc;
+    if (FP64) {
+        double tn, tf, sd;
+        rq_roots_disq64(double(RT_MAX), tn, tf, sd, double(c2), double(c1), double(c0));
+        t_near = float(tn); t_far = float(tf); sdisc = float(sd);
+    } else {
+        rq_roots_disq(RT_MAX, t_near, t_far, sdisc, c2, c1, c0);
+    }
+    float z_near = o.z + t_near * d.z;
+    float z_far = o.z + t_far * d.z;
+    t_near = z_near > z1 && z_near < z2 && t_near > t_min ? t_near : RT_MAX;
+    t_far = z_far > z1 && z_far < z2 && t_far > t_min ? t_far : RT_MAX;
+    float t = fminf(fminf(t_near, t_far), fminf(t_cap1, t_cap2));
+    bool valid = t > t_min && t < RT_MAX;
+    if (valid) {
+        if (t == t_cap1 || t == t_cap2) {
+            is.x = 0.f; is.y = 0.f; is.z = t == t_cap2 ? 1.f : -1.f;
+        } else {
+            f3 nn = normalize(mk(o.x + t * d.x, o.y + t * d.y, (z0 - (o.z + t * d.z)) * tth2));
+            is.x = nn.x; is.y = nn.y; is.z = nn.z;
+        }
+        is.w = t;
+    }
+    return valid;
+}
+
+// csg_intersect_leaf_convexpolyhedron.h:22-106
+OXD bool leaf_convexpolyhedron(Is& is, const NodeV& n, const float4* plan, float t_min, f3 o, f3 d) {
+    const float INF = __int_as_float(0x7f800000);
+    float t0 = -INF, t1 = INF;
+    f3 t0n = mk(0.f, 0.f, 0.f), t1n = mk(0.f, 0.f, 0.f);
+    unsigned planeIdx = __float_as_uint(n.q0.x), planeNum = __float_as_uint(n.q0.y);
+    for (unsigned i = 0; i < planeNum; i++) {
+        float4 pl = plan[planeIdx + i];
+        f3 nn = mk(pl.x, pl.y, pl.z);
+        float nd = dot(nn, d);
+        float no = dot(nn, o);
+        float dist = no - pl.w;
+        float tc = -dist / nd;
+        bool parallel_inside = nd == 0.f && dist < 0.f;
+        bool parallel_outside = nd == 0.f && dist > 0.f;
+        if (parallel_inside) continue;
+        if (parallel_outside) return false;
+        if (nd < 0.f) { if (tc > t0) { t0 = tc; t0n = nn; } }
+        else { if (tc < t1) { t1 = tc; t1n = nn; } }
+    }
+    bool valid = t0 < t1;
+    if (valid) {
+        if (t0 > t_min) { is.x = t0n.x; is.y = t0n.y; is.z = t0n.z; is.w = t0; }
+        else if (t1 > t_min) { is.x = t1n.x; is.y = t1n.y; is.z = t1n.z; is.w = t1; }
+    }
+    return valid;
+}
+
+// csg_intersect_leaf_phicut.h:581-635
+OXD bool leaf_phicut(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
+    float cosPhi0 = n.q0.x, sinPhi0 = n.q0.y, cosPhi1 = n.q0.z, sinPhi1 = n.q0.w;
+    float d_n0 = d.x * sinPhi0 + d.y * (-cosPhi0);
+    float d_n1 = d.x * (-sinPhi1) + d.y * (cosPhi1);
+    float o_n0 = o.x * sinPhi0 + o.y * (-cosPhi0);
+    float o_n1 = o.x * (-sinPhi1) + o.y * (cosPhi1);
+    float t0 = d_n0 == 0.f ? t_min : -o_n0 / d_n0;
+    float t1 = d_n1 == 0.f ? t_min : -o_n1 / d_n1;
+    float PR = d_n0 == 0.f ? -o_n0 : -d_n0;
+    float QR = d_n1 == 0.f ? -o_n1 : -d_n1;
+    float PQ = cosPhi0 * sinPhi1 - cosPhi1 * sinPhi0;
+    bool unbounded_exit = PQ >= 0.f ? (PR >= 0.f && QR <= 0.f) : (PR >= 0.f || QR <= 0.f);
+    float side0 = o.x * cosPhi0 + o.y * sinPhi0 + (d.x * cosPhi0 + d.y * sinPhi0) * t0;
+    float side1 = o.x * cosPhi1 + o.y * sinPhi1 + (d.x * cosPhi1 + d.y * sinPhi1) * t1;
+    if (side0 < 0.f) t0 = t_min;
+    if (side1 < 0.f) t1 = t_min;
+    float t_near = fminf(t0, t1), t_far = fmaxf(t0, t1);
+    float t = t_near > t_min ? t_near : (t_far > t_min ? t_far : t_min);
+    bool valid = t > t_min;
+    if (valid) {
+        is.x = t == t1 ? -sinPhi1 : sinPhi0;
+        is.y = t == t1 ? cosPhi1 : -cosPhi0;
+        is.z = 0.f;
+        is.w = t;
+    } else if (unbounded_exit) {
+        is.y = -is.y;
+    }
+    return valid;
+}
+
+// halfspace: plane p.n = w, inside p.n < w, exit at infinity signalled in isect.y
+OXD bool leaf_halfspace(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
+    f3 nn = mk(n.q0.x, n.q0.y, n.q0.z);
+    float w = n.q0.w;
+    float on = dot(o, nn), dn = dot(d, nn);
+    float t = (w - on) / dn;
+    bool valid = t > t_min && dn * (on - w) < 0.f;
+    if (valid) { is.x = nn.x; is.y = nn.y; is.z = nn.z; is.w = t; }
+    else if (dn <= 0.f && on - w <= 0.f) { is.y = -is.y; }
+    return valid;
+}
+
+// intersect_leaf (CSG/csg_intersect_leaf.h:175-324)
+template <bool FP64>
+OXD bool intersect_leaf(Is& is, const float4* node, int ni, const float4* plan, const float4* itra,
+                        float t_min, f3 ro, f3 rd) {
+    NodeV n = load_node(node, ni);
+    is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
+    unsigned tc = typecode(n);
+    unsigned gtc = __float_as_uint(n.q3.w);
+    unsigned gt = gtc & 0x7fffffffu;
+    bool complement = (gtc & 0x80000000u) != 0;
+    f3 o = ro, d = rd;
+    float4 q0, q1, q2, q3;
+    if (gt > 0) {
+        const float4* q = itra + 4 * (gt - 1);
+        q0 = q[0]; q1 = q[1]; q2 = q[2]; q3 = q[3];
+        o = mk(q0.x * ro.x + q1.x * ro.y + q2.x * ro.z + q3.x * 1.f,
+               q0.y * ro.x + q1.y * ro.y + q2.y * ro.z + q3.y * 1.f,
+               q0.z * ro.x + q1.z * ro.y + q2.z * ro.z + q3.z * 1.f);
+        d = mk(q0.x * rd.x + q1.x * rd.y + q2.x * rd.z + q3.x * 0.f,
+               q0.y * rd.x + q1.y * rd.y + q2.y * rd.z + q3.y * 0.f,
+               q0.z * rd.x + q1.z * rd.y + q2.z * rd.z + q3.z * 0.f);
+    }
+    bool valid = false;
+    switch (tc) {
+        case CSG_SPHERE: valid = leaf_sphere<FP64>(is, n, t_min, o, d); break;
+        case CSG_ZSPHERE: valid = leaf_zsphere<FP64>(is, n, t_min, o, d); break;
+        case CSG_CYLINDER: valid = leaf_cylinder<FP64>(is, n, t_min, o, d); break;
+        case CSG_BOX3: valid = leaf_box3(is, n, t_min, o, d); break;
+        case CSG_CONE: valid = leaf_cone<FP64>(is, n, t_min, o, d); break;
+        case CSG_CONVEXPOLYHEDRON: valid = leaf_convexpolyhedron(is, n, plan, t_min, o, d); break;
+        case CSG_PHICUT: valid = leaf_phicut(is, n, t_min, o, d); break;
+        case CSG_HALFSPACE: valid = leaf_halfspace(is, n, t_min, o, d); break;
+        default: break;
+    }
+    if (valid && gt > 0) {
+        float x = q0.x * is.x + q0.y * is.y + q0.z * is.z;
+        float y = q1.x * is.x + q1.y * is.y + q1.z * is.z;
+        float z = q2.x * is.x + q2.y * is.y + q2.z * is.z;
+        is.x = x; is.y = y; is.z = z;
+    }
+    if (complement) {
+        is.x = valid ? -is.x : -0.f;
+        is.y = valid ? -is.y : is.y;
+        is.z = valid ? -is.z : is.z;
+    }
+    return valid;
+}
+
+// Kensler single-hit CSG decision table, csg_classify.h:86-101
+OXD int csg_lut(unsigned op, int a, int b, bool acloser) {
+    const uint32_t AC = op == 1 ? 0x22121141u : (op == 2 ? 0x00014014u : 0x00141141u);
+    const uint32_t BC = op == 1 ? 0x22115122u : (op == 2 ? 0x00022055u : 0x00133155u);
+    unsigned offset = 3u * unsigned(a) + unsigned(b);
+    uint32_t t = acloser ? AC : BC;
+    return offset < 8 ? int((t >> (offset * 4)) & 0xf) : 0;
+}
+
+OXD int classify(const Is& ise, f3 dir, float tmin) {
+    return fabsf(ise.w) > tmin ? ((ise.x * dir.x + ise.y * dir.y + ise.z * dir.z) < 0.f ? 0 : 1) : 2;
+}
+
+// intersect_tree (CSG/csg_intersect_tree.h:277-672): postorder evaluation with a tranche
+// stack of subtree slices and a stack of partial intersects; side recorded in sign(t).
+template <bool FP64>
+OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* plan, const float4* itra,
+                        float t_min, f3 ro, f3 rd) {
+    const unsigned numNode = __float_as_uint(node[4 * root].x);
+    const unsigned height = unsigned(__ffs(int(numNode + 1)) - 2);
+    const float propagate_epsilon = 0.0001f;
+    float tr_tmin[4];
+    unsigned tr_slice[4];
+    int tr = -1;
+    Is st[15];
+    int sp = -1;
+    int ierr = 0;
+    tr++; tr_slice[tr] = ((1u << height) & 0xffu) << 16; tr_tmin[tr] = t_min;
+    while (tr > -1) {
+        unsigned slice = tr_slice[tr];
+        float tmin = tr_tmin[tr];
+        tr--;
+        unsigned nodeIdx = (slice >> 16) & 0xff, endIdx = (slice >> 24) & 0xff;
+        while (nodeIdx != endIdx) {
+            unsigned depth = 31u - unsigned(__clz(int(nodeIdx)));
+            unsigned elevation = height - depth;
+            int ni = root + int(nodeIdx) - 1;
+            unsigned tc = __float_as_uint(node[4 * ni + 3].z);
+            if (tc == CSG_ZERO) {
+                nodeIdx = (nodeIdx & 1u) ? nodeIdx >> 1 : (nodeIdx << elevation) + (1u << elevation);
+                continue;
+            }
+            if (tc >= CSG_NODE) {
+                Is ni_is;
+                intersect_leaf<FP64>(ni_is, node, ni, plan, itra, tmin, ro, rd);
+                ni_is.w = copysignf(ni_is.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
+                if (sp >= 14) { ierr = 1; break; }
+                st[++sp] = ni_is;
+            } else {
+                if (sp < 1) { ierr = 1; break; }
+                bool firstLeft = signbit(st[sp].w);
+                bool secondLeft = signbit(st[sp - 1].w);
+                if (!(firstLeft ^ secondLeft)) { ierr = 1; break; }
+                int left = firstLeft ? sp : sp - 1;
+                int right = firstLeft ? sp - 1 : sp;
+                Is L = st[left], R = st[right];
+                int l_state = classify(L, rd, tmin);
+                int r_state = classify(R, rd, tmin);
+                float t_left = fabsf(L.w), t_right = fabsf(R.w);
+                bool leftIsCloser = t_left <= t_right;
+                bool l_promote = l_state == 2 && (signbit(L.x) || signbit(L.y));
+                bool r_promote = r_state == 2 && (signbit(R.x) || signbit(R.y));
+                if (r_promote) { r_state = 1; leftIsCloser = true; }
+                if (l_promote) { l_state = 1; leftIsCloser = false; }
+                int ctrl = csg_lut(tc, l_state, r_state, leftIsCloser);
+                if (ctrl < 4) {
+                    Is result;
+                    if (ctrl == 0) { result.x = 0.f; result.y = 0.f; result.z = 0.f; result.w = 0.f; }
+                    else result = ctrl == 1 ? L : R;
+                    if (ctrl == 3) { result.x = -result.x; result.y = -result.y; result.z = -result.z; }
+                    result.w = copysignf(result.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
+                    sp -= 2;
+                    st[++sp] = result;
+                } else {
+                    bool loopA = ctrl == 4;
+                    Is loopside = loopA ? L : R;
+                    Is other = loopA ? R : L;
+                    unsigned leftIdx = 2 * nodeIdx, rightIdx = leftIdx + 1;
+                    float tminAdvanced = fabsf(loopside.w) + propagate_epsilon;
+                    sp -= 2;
+                    st[++sp] = other;
+                    unsigned endTree = ((nodeIdx & 0xffu) << 16) | ((endIdx & 0xffu) << 24);
+                    unsigned leftTree = (((leftIdx << (elevation - 1)) & 0xffu) << 16) | (((rightIdx << (elevation - 1)) & 0xffu) << 24);
+                    unsigned rightTree = (((rightIdx << (elevation - 1)) & 0xffu) << 16) | ((nodeIdx & 0xffu) << 24);
+                    unsigned loopTree = loopA ? leftTree : rightTree;
+                    if (tr >= 3) { ierr = 1; break; }
+                    tr++; tr_slice[tr] = endTree; tr_tmin[tr] = tmin;
+                    if (tr >= 3) { ierr = 1; break; }
+                    tr++; tr_slice[tr] = loopTree; tr_tmin[tr] = tminAdvanced;
+                    break;
+                }
+            }
+            nodeIdx = (nodeIdx & 1u) ? nodeIdx >> 1 : (nodeIdx << elevation) + (1u << elevation);
+        }
+        if (ierr) break;
+    }
+    out.x = 0.f; out.y = 0.f; out.z = 0.f; out.w = 0.f;
+    if (sp == 0) out = st[0];
+    return out.w > 0.f;
+}
+
+// intersect_prim (CSG/csg_intersect_tree.h:684-719)
+template <bool FP64>
+OXD bool intersect_prim(Is& is, const float4* node, int root, const float4* plan, const float4* itra,
+                        float t_min, f3 ro, f3 rd) {
+    unsigned tc = __float_as_uint(node[4 * root + 3].z);
+    if (tc >= CSG_LEAF) return intersect_leaf<FP64>(is, node, root, plan, itra, t_min, ro, rd);
+    if (tc < CSG_NODE) return intersect_tree<FP64>(is, node, root, plan, itra, t_min, ro, rd);
+    is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
+    return false;
+}
+
+} // namespace oxd

@@ -1,0 +1,745 @@
+// oxr_engine.cpp -- C-ABI implementation: geometry/table upload with BVH builds,
+// per-event genstep upload and seeding, launches, hit compaction, gathers, queries and
+// the autotuner. The serving loop lives in oxr_server.cpp.
+#include "oxr_engine.h"
+#include "oxr_bvh.h"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace oxh {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+int fail(int code, const std::string& m) { g_err = m; return code; }
+int hip_fail(hipError_t e, const char* what) {
+    std::ostringstream ss;
+    ss << "HIP error " << int(e) << " (" << hipGetErrorString(e) << ") in " << what;
+    g_err = ss.str();
+    return OXR_EHIP;
+}
+
+void EventBuffers::release_all() {
+    gs.release(); gs_off.release(); seed.release(); photon.release(); seq.release(); record.release();
+    hitflag.release(); hit.release(); counts.release(); offs.release(); counters.release();
+    input.release(); pre.release();
+}
+
+static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+oxd::Geom Engine::geom() const {
+    oxd::Geom g;
+    g.node = node.p; g.plan = plan.p; g.itra = itra.p; g.prim = prim.p;
+    g.tlas = tlas.p; g.blas = blas.p; g.inst = inst.p;
+    g.tlas_root = tlas_root; g.num_inst = num_inst;
+    return g;
+}
+
+int Engine::stack_cap() const { return std::max(4, stack_need + 2); }
+
+int Engine::ntop_for(int block) const {
+    int want = std::max(0, tune.lds_top_nodes);
+    want = int(std::min<int64_t>(want, tlas_nodes));
+    size_t lim = 64 * 1024;
+    size_t stack = size_t(block) * stack_cap() * sizeof(int);
+    if (stack >= lim) return 0;
+    size_t room = (lim - stack) / sizeof(oxd::BNode);
+    return int(std::min<size_t>(size_t(want), room));
+}
+
+oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int ntop) const {
+    oxd::Params P{};
+    P.g = geom();
+    P.t = tables;
+    P.gs = b.gs.p; P.seed = b.seed.p;
+    P.input = nullptr; P.pre = nullptr; P.npre = 0;
+    P.photon = b.photon.p;
+    P.seq = cfg.seq_enabled ? b.seq.p : nullptr;
+    P.record = cfg.max_record > 0 ? b.record.p : nullptr;
+    P.max_record = cfg.max_record;
+    P.hitflag = b.hitflag.p;
+    P.bounce_out = nullptr; P.overflow_out = nullptr;
+    P.counters = b.counters.p;
+    P.num_photon = n;
+    P.slot_offset = uint64_t(slot_off);
+    P.event_index = evt_idx;
+    P.max_bounce = cfg.max_bounce;
+    P.tmin = cfg.tmin; P.tmin0 = cfg.tmin0; P.eps0_mask = cfg.eps0_mask;
+    P.tmax = cfg.tmax; P.max_time = cfg.max_time; P.hitmask = cfg.hitmask;
+    P.rng_seed = cfg.rng_seed; P.rng_offset = cfg.rng_offset; P.skip = cfg.skipahead_event;
+    P.aligned = cfg.rng_schedule == OXR_RNG_ALIGNED ? 1 : 0;
+    P.ntop = ntop;
+    P.stack_cap = stack_cap();
+    P.refill_threshold = std::max(1, tune.refill_batch);
+    P.segs = nullptr;
+    P.nseg = 0;
+    return P;
+}
+
+int ensure_event_capacity(Engine& e, EventBuffers& b, int64_t n) {
+    size_t N = size_t(std::max<int64_t>(n, 1));
+    OXR_HIP(b.seed.reserve(N));
+    OXR_HIP(b.photon.reserve(4 * N));
+    if (e.cfg.seq_enabled) OXR_HIP(b.seq.reserve(2 * N));
+    if (e.cfg.max_record > 0) OXR_HIP(b.record.reserve(4 * N * size_t(e.cfg.max_record)));
+    OXR_HIP(b.hitflag.reserve(N));
+    OXR_HIP(b.hit.reserve(4 * N));
+    int nb = oxl::compact_blocks(int64_t(N));
+    OXR_HIP(b.counts.reserve(size_t(nb)));
+    OXR_HIP(b.offs.reserve(size_t(nb) + 1));
+    OXR_HIP(b.counters.reserve(8));
+    return OXR_OK;
+}
+
+// validate gensteps against the engine's tables and form the photon prefix offsets
+int genstep_offsets(Engine& e, const float* gs, int64_t num_gs, std::vector<long long>& off, int64_t* num_photon) {
+    if (num_gs < 0 || (num_gs > 0 && !gs)) return fail(OXR_EINVAL, "gensteps: null pointer or negative count");
+    off.assign(size_t(num_gs) + 1, 0);
+    for (int64_t g = 0; g < num_gs; g++) {
+        const float* q = gs + 24 * g;
+        uint32_t code = fbits(q[0]);
+        uint32_t np = fbits(q[3]);
+        if (code == 15 || code == 18 || code == 1) {
+            uint32_t matline = fbits(q[2]);
+            if (int64_t(matline) >= 4 * int64_t(e.num_bnd))
+                return fail(OXR_EINVAL, "Cerenkov genstep matline outside the boundary table");
+        }
+        off[size_t(g) + 1] = off[size_t(g)] + (long long)np;
+    }
+    *num_photon = off[size_t(num_gs)];
+    return OXR_OK;
+}
+
+int upload_gensteps(Engine& e, EventBuffers& b, hipStream_t s, const float* gs, int64_t num_gs, int64_t* num_photon) {
+    std::vector<long long> off;
+    int64_t n = 0;
+    int rc0 = genstep_offsets(e, gs, num_gs, off, &n);
+    if (rc0) return rc0;
+    if (n > e.cfg.max_photon) {
+        std::ostringstream ss;
+        ss << "slice of " << n << " photons exceeds max_photon " << e.cfg.max_photon;
+        return fail(OXR_EINVAL, ss.str());
+    }
+    int rc = ensure_event_capacity(e, b, n);
+    if (rc) return rc;
+    OXR_HIP(b.gs.reserve(size_t(std::max<int64_t>(num_gs, 1)) * 6));
+    OXR_HIP(b.gs_off.reserve(size_t(num_gs) + 1));
+    if (num_gs > 0) {
+        OXR_HIP(hipMemcpyAsync(b.gs.p, gs, size_t(num_gs) * 96, hipMemcpyHostToDevice, s));
+        OXR_HIP(hipMemcpyAsync(b.gs_off.p, off.data(), (size_t(num_gs) + 1) * sizeof(long long), hipMemcpyHostToDevice, s));
+        OXR_HIP(oxl::fill_seed(s, b.gs_off.p, int(num_gs), b.seed.p, n));
+    }
+    OXR_HIP(hipStreamSynchronize(s));
+    *num_photon = n;
+    return OXR_OK;
+}
+
+int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slot_off, int32_t evt_idx,
+               int form, bool count, oxr_launch_stats* st, const float* pre, int npre, const float4* input) {
+    auto t0 = std::chrono::steady_clock::now();
+    int block = form == OXR_FORM_PERSISTENT ? e.tune.block_size : 256;
+    if (block < 64 || block > 1024 || block % 64) return fail(OXR_EINVAL, "tuning block_size must be a multiple of 64 in [64,1024]");
+    int ntop = e.ntop_for(block);
+    oxd::Params P = e.params(b, n, slot_off, evt_idx, ntop);
+    P.pre = pre; P.npre = npre; P.input = input;
+    size_t smem = oxl::simulate_smem(P, block);
+    if (smem > 64 * 1024) return fail(OXR_EINVAL, "LDS budget exceeded: lower block_size or lds_top_nodes");
+    OXR_HIP(hipMemsetAsync(b.counters.p, 0, 8 * sizeof(unsigned long long), s));
+    OXR_HIP(hipEventRecord(e.e0, s));
+    if (form == OXR_FORM_PERSISTENT) {
+        int grid = std::max(1, e.tune.blocks_per_cu) * std::max(1, e.num_cu);
+        OXR_HIP(oxl::simulate_persist(s, P, block, grid, e.cfg.fp64_roots != 0));
+    } else {
+        OXR_HIP(oxl::simulate_plain(s, P, block, e.cfg.fp64_roots != 0, count));
+    }
+    OXR_HIP(hipEventRecord(e.e1, s));
+    OXR_HIP(oxl::compact(s, b.hitflag.p, b.photon.p, n, b.counts.p, b.offs.p, b.hit.p, n));
+    OXR_HIP(hipEventRecord(e.e2, s));
+    unsigned long long ctr[8] = {0};
+    long long nhit = 0;
+    OXR_HIP(hipMemcpyAsync(ctr, b.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s));
+    if (n > 0) OXR_HIP(hipMemcpyAsync(&nhit, b.offs.p + oxl::compact_blocks(n), sizeof(long long), hipMemcpyDeviceToHost, s));
+    OXR_HIP(hipStreamSynchronize(s));
+    if (ctr[4] != 0) return fail(OXR_EINVAL, "traversal stack overflow in BVH (geometry deeper than the stack)");
+    float k_ms = 0.f, c_ms = 0.f;
+    OXR_HIP(hipEventElapsedTime(&k_ms, e.e0, e.e1));
+    OXR_HIP(hipEventElapsedTime(&c_ms, e.e1, e.e2));
+    auto t1 = std::chrono::steady_clock::now();
+    if (st) {
+        st->num_photon = n;
+        st->num_hit = nhit;
+        st->bounce_steps = ctr[0];
+        st->kernel_ms = k_ms;
+        st->compact_ms = c_ms;
+        st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->node_visits = ctr[2];
+        st->prim_tests = ctr[3];
+    }
+    e.num_hit = nhit;
+    return OXR_OK;
+}
+
+// inverse of the instance transform in double (row convention, identity column ignored)
+static void invert_instance(const float* q, double W[4][3]) {
+    double a[3][3], t[3];
+    for (int r = 0; r < 3; r++) for (int k = 0; k < 3; k++) a[r][k] = q[4 * r + k];
+    for (int k = 0; k < 3; k++) t[k] = q[12 + k];
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double inv[3][3];
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int i = 0; i < 3; i++) for (int k = 0; k < 3; k++) W[i][k] = inv[i][k];
+    for (int k = 0; k < 3; k++) W[3][k] = -(t[0] * inv[0][k] + t[1] * inv[1][k] + t[2] * inv[2][k]);
+}
+
+static bool supported_leaf(uint32_t tc) {
+    return tc == 101 || tc == 103 || tc == 105 || tc == 108 || tc == 110 || tc == 112 || tc == 121 || tc == 125;
+}
+
+static int validate_and_prims(Engine& e, const oxr_geometry* g, const oxr_physics* ph, std::vector<int2>& primtab) {
+    const uint32_t* nu = reinterpret_cast<const uint32_t*>(g->node);
+    auto node_u = [&](int64_t i, int k) { return nu[16 * i + k]; };
+    primtab.resize(size_t(g->num_prim));
+    for (int64_t p = 0; p < g->num_prim; p++) {
+        const int32_t* pi = reinterpret_cast<const int32_t*>(g->prim + 16 * p);
+        const uint32_t* pu = reinterpret_cast<const uint32_t*>(g->prim + 16 * p);
+        int32_t numNode = pi[0], nodeOffset = pi[1];
+        if (numNode < 1 || nodeOffset < 0 || int64_t(nodeOffset) + numNode > g->num_node)
+            return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": node range outside node array");
+        if (numNode > 255 || ((numNode + 1) & numNode) != 0)
+            return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": numNode must be 2^k-1 and <= 255");
+        int height = 0;
+        while ((1 << (height + 1)) - 1 < numNode) height++;
+        for (int i = 1; i <= numNode; i++) {
+            int64_t ni = nodeOffset + i - 1;
+            uint32_t tc = node_u(ni, 14);
+            int depth = 31 - __builtin_clz(unsigned(i));
+            bool op = tc == 1 || tc == 2 || tc == 3;
+            if (numNode == 1 && !supported_leaf(tc))
+                return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": unsupported leaf typecode " + std::to_string(tc));
+            if (numNode > 1) {
+                if (i == 1 && !op) return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": tree root must be an operator");
+                if (op && depth == height) return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": operator at leaf level");
+                if (!op && tc != 0 && !supported_leaf(tc))
+                    return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": unsupported node typecode " + std::to_string(tc));
+            }
+            if (tc != 0 && !op) {
+                uint32_t gt = node_u(ni, 15) & 0x7fffffffu;
+                if (int64_t(gt) > g->num_itra) return fail(OXR_EINVAL, "node transform index outside itra");
+                if (tc == 112) {
+                    uint32_t pidx = node_u(ni, 0), pnum = node_u(ni, 1);
+                    if (int64_t(pidx) + int64_t(pnum) > g->num_plan) return fail(OXR_EINVAL, "convexpolyhedron planes outside plan array");
+                }
+            }
+        }
+        if (numNode > 1 && int(node_u(nodeOffset, 0)) != numNode)
+            return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": root subNum differs from numNode");
+        uint32_t bnd = node_u(nodeOffset, 6);
+        if (int64_t(bnd) >= int64_t(ph->num_bnd)) return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": boundary outside bnd table");
+        uint32_t gp = pu[15];
+        primtab[size_t(p)] = make_int2(nodeOffset, int(((gp & 0xffffu) << 16) | (bnd & 0xffffu)));
+    }
+    return OXR_OK;
+}
+
+} // namespace oxh
+
+using namespace oxh;
+
+extern "C" {
+
+const char* oxr_version(void) { return "oxr 0.1 gfx950"; }
+const char* oxr_last_error(void) { return g_err.c_str(); }
+
+void oxr_config_defaults(oxr_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->max_bounce = 31;
+    c->tmin = 0.05f; c->tmin0 = 0.05f;
+    c->eps0_mask = (1u << 2) | (1u << 0) | (1u << 1) | (1u << 5) | (1u << 4);
+    c->tmax = 1e6f; c->max_time = 1e27f;
+    c->hitmask = 1u << 6;
+    c->rng_seed = 0; c->rng_offset = 0; c->skipahead_event = 100000;
+    c->rng_schedule = OXR_RNG_PRODUCTION;
+    c->fp64_roots = 0; c->seq_enabled = 0; c->max_record = 0;
+    c->max_photon = int64_t(1) << 24;
+}
+
+int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_config* cfg, oxr_engine** out) {
+    if (!g || !ph || !cfg || !out) return fail(OXR_EINVAL, "null argument");
+    *out = nullptr;
+    if (!g->solid || !g->prim || !g->node || !g->inst || g->num_solid < 1 || g->num_prim < 1 || g->num_node < 1 || g->num_inst < 1)
+        return fail(OXR_EINVAL, "geometry needs solid, prim, node and inst arrays");
+    if (g->num_itra > 0 && !g->itra) return fail(OXR_EINVAL, "itra pointer null");
+    if (g->num_plan > 0 && !g->plan) return fail(OXR_EINVAL, "plan pointer null");
+    if (!ph->bnd || !ph->optical || ph->num_bnd < 1 || ph->num_wl < 2 || !(ph->wl_step > 0.f))
+        return fail(OXR_EINVAL, "physics needs bnd and optical tables");
+    if (ph->icdf && (ph->icdf_rows < 1 || ph->icdf_width < 2)) return fail(OXR_EINVAL, "icdf shape");
+    if (ph->icdf && ph->hd_factor != 0 && ph->icdf_rows < 3) return fail(OXR_EINVAL, "hd_factor needs 3 icdf rows");
+    if (cfg->max_record < 0 || cfg->max_photon < 1) return fail(OXR_EINVAL, "config");
+    int dev = 0;
+    int ndev = 0;
+    OXR_HIP(hipGetDeviceCount(&ndev));
+    if (ndev < 1) return fail(OXR_EHIP, "no HIP device visible");
+    OXR_HIP(hipGetDevice(&dev));
+    Engine* e = new Engine();
+    e->device = dev;
+    e->cfg = *cfg;
+    hipDeviceProp_t prop;
+    hipError_t he = hipGetDeviceProperties(&prop, dev);
+    if (he != hipSuccess) { delete e; return hip_fail(he, "hipGetDeviceProperties"); }
+    e->num_cu = prop.multiProcessorCount;
+    e->name = prop.name;
+    e->arch = prop.gcnArchName;
+    e->tune.block_size = 256;
+    e->tune.blocks_per_cu = 3;
+    e->tune.lds_top_nodes = 64;
+    e->tune.refill_batch = 16;
+    auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
+    if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
+    if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
+    if ((he = hipEventCreate(&e->e1)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
+    if ((he = hipEventCreate(&e->e2)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
+
+    std::vector<int2> primtab;
+    int rc = validate_and_prims(*e, g, ph, primtab);
+    if (rc) return bail(rc);
+    e->num_prim = int(g->num_prim); e->num_node = int(g->num_node); e->num_solid = int(g->num_solid);
+    e->num_inst = int(g->num_inst);
+    e->num_bnd = ph->num_bnd;
+
+    // ---- per-solid BLAS over prim AABBs
+    std::vector<oxh::NodeOut> blas_all;
+    int max_blas_depth = 0;
+    e->solid_prim_offset.resize(size_t(g->num_solid));
+    e->solid_num_prim.resize(size_t(g->num_solid));
+    e->solid_blas_root.resize(size_t(g->num_solid));
+    std::vector<oxh::Box> solid_box(static_cast<size_t>(g->num_solid));
+    for (int64_t s = 0; s < g->num_solid; s++) {
+        const int32_t* so = g->solid + 12 * s;
+        int numPrim = so[4], primOffset = so[5];
+        if (numPrim < 1 || primOffset < 0 || int64_t(primOffset) + numPrim > g->num_prim)
+            return bail(fail(OXR_EINVAL, "solid " + std::to_string(s) + ": prim range outside prim array"));
+        std::vector<oxh::Box> boxes(static_cast<size_t>(numPrim));
+        oxh::Box sb = oxh::Box::empty();
+        for (int p = 0; p < numPrim; p++) {
+            const float* q = g->prim + 16 * size_t(primOffset + p);
+            oxh::Box b;
+            b.lo[0] = q[8]; b.lo[1] = q[9]; b.lo[2] = q[10];
+            b.hi[0] = q[11]; b.hi[1] = q[12]; b.hi[2] = q[13];
+            for (int k = 0; k < 3; k++) if (!(b.lo[k] <= b.hi[k])) return bail(fail(OXR_EINVAL, "prim with empty/NaN AABB"));
+            boxes[size_t(p)] = b;
+            sb.grow(b);
+        }
+        oxh::Bvh bv = oxh::build_bvh(boxes, int(blas_all.size()));
+        e->solid_prim_offset[size_t(s)] = primOffset;
+        e->solid_num_prim[size_t(s)] = numPrim;
+        e->solid_blas_root[size_t(s)] = bv.root_ref;
+        max_blas_depth = std::max(max_blas_depth, bv.depth);
+        blas_all.insert(blas_all.end(), bv.nodes.begin(), bv.nodes.end());
+        solid_box[size_t(s)] = sb;
+    }
+    // ---- instances and TLAS
+    std::vector<oxd::InstD> insts(static_cast<size_t>(g->num_inst));
+    std::vector<oxh::Box> inst_box(static_cast<size_t>(g->num_inst));
+    for (int64_t i = 0; i < g->num_inst; i++) {
+        const float* q = g->inst + 16 * i;
+        int32_t gas, sid1;
+        std::memcpy(&gas, q + 7, 4);
+        std::memcpy(&sid1, q + 11, 4);
+        if (gas < 0 || gas >= g->num_solid) return bail(fail(OXR_EINVAL, "instance " + std::to_string(i) + ": gas_idx outside solids"));
+        bool ident = q[0] == 1.f && q[1] == 0.f && q[2] == 0.f && q[4] == 0.f && q[5] == 1.f && q[6] == 0.f &&
+                     q[8] == 0.f && q[9] == 0.f && q[10] == 1.f && q[12] == 0.f && q[13] == 0.f && q[14] == 0.f;
+        double W[4][3];
+        invert_instance(q, W);
+        for (int r = 0; r < 4; r++) for (int k = 0; k < 3; k++)
+            if (!std::isfinite(W[r][k])) return bail(fail(OXR_EINVAL, "instance transform not invertible"));
+        oxd::InstD d;
+        d.c0 = make_float4(float(W[0][0]), float(W[1][0]), float(W[2][0]), float(W[3][0]));
+        d.c1 = make_float4(float(W[0][1]), float(W[1][1]), float(W[2][1]), float(W[3][1]));
+        d.c2 = make_float4(float(W[0][2]), float(W[1][2]), float(W[2][2]), float(W[3][2]));
+        uint32_t ii_id = ((uint32_t(i) & 0xffffu) << 16) | (uint32_t(sid1) & 0xffffu);
+        d.info = make_int4(e->solid_prim_offset[size_t(gas)], int(ii_id), ident ? 1 : 0, e->solid_blas_root[size_t(gas)]);
+        insts[size_t(i)] = d;
+        // world box of the instance: solid box corners through the object->world transform
+        const oxh::Box& sb = solid_box[size_t(gas)];
+        oxh::Box wb = oxh::Box::empty();
+        for (int c = 0; c < 8; c++) {
+            double p[3] = {(c & 1) ? sb.hi[0] : sb.lo[0], (c & 2) ? sb.hi[1] : sb.lo[1], (c & 4) ? sb.hi[2] : sb.lo[2]};
+            double w[3];
+            for (int k = 0; k < 3; k++) w[k] = p[0] * q[k] + p[1] * q[4 + k] + p[2] * q[8 + k] + q[12 + k];
+            wb.grow_point(w);
+        }
+        inst_box[size_t(i)] = wb;
+    }
+    oxh::Bvh tl = oxh::build_bvh(inst_box, 0);
+    e->tlas_root = tl.root_ref;
+    e->tlas_nodes = int64_t(tl.nodes.size());
+    e->blas_nodes = int64_t(blas_all.size());
+    e->stack_need = tl.depth + 1 + max_blas_depth;
+    if (e->stack_cap() > 96) return bail(fail(OXR_EINVAL, "BVH too deep for the traversal stack"));
+
+    // ---- uploads
+    auto up = [&](auto& buf, const void* src, size_t count, size_t elem) -> int {
+        size_t nn = std::max<size_t>(count, 1);
+        hipError_t x = buf.reserve(nn);
+        if (x != hipSuccess) return hip_fail(x, "hipMalloc geometry");
+        if (count) {
+            x = hipMemcpy(buf.p, src, count * elem, hipMemcpyHostToDevice);
+            if (x != hipSuccess) return hip_fail(x, "hipMemcpy geometry");
+        }
+        e->geometry_bytes += int64_t(nn * elem);
+        return OXR_OK;
+    };
+    if ((rc = up(e->node, g->node, size_t(g->num_node) * 4, 16))) return bail(rc);
+    if ((rc = up(e->plan, g->plan, size_t(g->num_plan), 16))) return bail(rc);
+    if ((rc = up(e->itra, g->itra, size_t(g->num_itra) * 4, 16))) return bail(rc);
+    if ((rc = up(e->prim, primtab.data(), primtab.size(), 8))) return bail(rc);
+    if ((rc = up(e->tlas, tl.nodes.data(), tl.nodes.size(), 64))) return bail(rc);
+    if ((rc = up(e->blas, blas_all.data(), blas_all.size(), 64))) return bail(rc);
+    if ((rc = up(e->inst, insts.data(), insts.size(), 64))) return bail(rc);
+    size_t bnd_f4 = size_t(ph->num_bnd) * 4 * 2 * size_t(ph->num_wl);
+    if ((rc = up(e->bnd, ph->bnd, bnd_f4, 16))) return bail(rc);
+    if ((rc = up(e->optical, ph->optical, size_t(ph->num_bnd) * 4, 16))) return bail(rc);
+    if (ph->icdf) { if ((rc = up(e->icdf, ph->icdf, size_t(ph->icdf_rows) * ph->icdf_width, 4))) return bail(rc); }
+    e->tables.bnd = e->bnd.p;
+    e->tables.optical = e->optical.p;
+    e->tables.icdf = ph->icdf ? e->icdf.p : nullptr;
+    e->tables.num_wl = ph->num_wl;
+    e->tables.wl_min = ph->wl_min;
+    e->tables.wl_step = ph->wl_step;
+    e->tables.icdf_width = ph->icdf ? ph->icdf_width : 0;
+    e->tables.hd_factor = ph->icdf ? ph->hd_factor : 0;
+    if ((he = hipDeviceSynchronize()) != hipSuccess) return bail(hip_fail(he, "upload sync"));
+    *out = reinterpret_cast<oxr_engine*>(e);
+    return OXR_OK;
+}
+
+void oxr_engine_destroy(oxr_engine* eh) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    e->ev.release_all();
+    e->node.release(); e->plan.release(); e->itra.release(); e->bnd.release(); e->prim.release();
+    e->tlas.release(); e->blas.release(); e->inst.release(); e->optical.release(); e->icdf.release();
+    if (e->e0) (void)hipEventDestroy(e->e0);
+    if (e->e1) (void)hipEventDestroy(e->e1);
+    if (e->e2) (void)hipEventDestroy(e->e2);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int oxr_engine_set_config(oxr_engine* eh, const oxr_config* c) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !c) return fail(OXR_EINVAL, "null argument");
+    if (c->max_record < 0 || c->max_photon < 1) return fail(OXR_EINVAL, "config");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->cfg = *c;
+    e->launched = false;
+    return OXR_OK;
+}
+
+int oxr_engine_set_tuning(oxr_engine* eh, const oxr_tuning* t) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !t) return fail(OXR_EINVAL, "null argument");
+    if (t->block_size < 64 || t->block_size > 1024 || t->block_size % 64 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
+        t->lds_top_nodes < 0 || t->refill_batch < 1)
+        return fail(OXR_EINVAL, "tuning out of range");
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->tune = *t;
+    return OXR_OK;
+}
+
+int oxr_engine_get_tuning(oxr_engine* eh, oxr_tuning* t) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !t) return fail(OXR_EINVAL, "null argument");
+    *t = e->tune;
+    return OXR_OK;
+}
+
+int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !info) return fail(OXR_EINVAL, "null argument");
+    std::memset(info, 0, sizeof(*info));
+    info->num_solid = e->num_solid; info->num_prim = e->num_prim; info->num_node = e->num_node; info->num_inst = e->num_inst;
+    info->tlas_nodes = e->tlas_nodes; info->blas_nodes = e->blas_nodes;
+    info->node_bytes = int32_t(sizeof(oxd::BNode));
+    info->num_cu = e->num_cu;
+    info->device_bytes = e->geometry_bytes;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) { info->free_bytes = int64_t(fr); info->total_bytes = int64_t(tot); }
+    std::strncpy(info->device_name, e->name.c_str(), sizeof(info->device_name) - 1);
+    std::strncpy(info->arch, e->arch.c_str(), sizeof(info->arch) - 1);
+    return OXR_OK;
+}
+
+int oxr_event_set_gensteps(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_index,
+                           uint64_t photon_slot_offset, int64_t* num_photon) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !num_photon) return fail(OXR_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int64_t n = 0;
+    int rc = upload_gensteps(*e, e->ev, e->stream, gs, num_gs, &n);
+    if (rc) return rc;
+    e->h_gs.assign(gs, gs + 24 * num_gs);
+    e->num_gs = num_gs;
+    e->num_photon = n;
+    e->event_index = event_index;
+    e->slot_offset = photon_slot_offset;
+    e->launched = false;
+    *num_photon = n;
+    return OXR_OK;
+}
+
+int oxr_event_set_input_photons(oxr_engine* eh, const float* photons, int64_t n) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || n < 0 || (n > 0 && !photons)) return fail(OXR_EINVAL, "bad input photons");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (n == 0) { e->have_input = false; e->input_n = 0; return OXR_OK; }
+    OXR_HIP(e->ev.input.reserve(size_t(n) * 4));
+    OXR_HIP(hipMemcpy(e->ev.input.p, photons, size_t(n) * 64, hipMemcpyHostToDevice));
+    e->have_input = true;
+    e->input_n = n;
+    return OXR_OK;
+}
+
+int oxr_event_set_precooked(oxr_engine* eh, const float* u, int64_t n_photon, int32_t n_per) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return fail(OXR_EINVAL, "null engine");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!u || n_photon == 0) { e->have_pre = false; e->npre = 0; return OXR_OK; }
+    if (n_photon < 0 || n_per < 1) return fail(OXR_EINVAL, "precooked shape");
+    OXR_HIP(e->ev.pre.reserve(size_t(n_photon) * size_t(n_per)));
+    OXR_HIP(hipMemcpy(e->ev.pre.p, u, size_t(n_photon) * size_t(n_per) * 4, hipMemcpyHostToDevice));
+    e->have_pre = true;
+    e->npre = n_per;
+    e->pre_rows = n_photon;
+    return OXR_OK;
+}
+
+static int check_sources(Engine& e, int64_t n, uint64_t slot_off) {
+    // INPUT_PHOTON gensteps read input[slot_offset + idx]; precooked rows read pre[idx]
+    bool any_input = false;
+    for (int64_t g = 0; g < e.num_gs; g++) if (fbits(e.h_gs[size_t(24 * g)]) == 19u) any_input = true;
+    if (any_input && (!e.have_input || int64_t(slot_off) + n > e.input_n))
+        return fail(OXR_ESTATE, "INPUT_PHOTON gensteps need oxr_event_set_input_photons covering every photon");
+    if (e.have_pre && e.pre_rows < n)
+        return fail(OXR_ESTATE, "precooked stream has fewer rows than photons");
+    return OXR_OK;
+}
+
+int oxr_simulate_launch(oxr_engine* eh, int32_t form, oxr_launch_stats* st) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return fail(OXR_EINVAL, "null engine");
+    if (form != OXR_FORM_PLAIN && form != OXR_FORM_PERSISTENT) return fail(OXR_EINVAL, "unknown form");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->num_gs == 0 && e->num_photon == 0) {
+        if (st) std::memset(st, 0, sizeof(*st));
+        e->launched = true;
+        e->num_hit = 0;
+        return OXR_OK;
+    }
+    int rc = check_sources(*e, e->num_photon, e->slot_offset);
+    if (rc) return rc;
+    rc = run_launch(*e, e->ev, e->stream, e->num_photon, int64_t(e->slot_offset), e->event_index, form, false, st,
+                    e->have_pre ? e->ev.pre.p : nullptr, e->have_pre ? e->npre : 0,
+                    e->have_input ? e->ev.input.p : nullptr);
+    if (rc == OXR_OK) e->launched = true;
+    return rc;
+}
+
+// Counting launch (node visits / prim tests) on the current event: test and
+// measurement helper, not a separate form.
+int oxr_simulate_launch_counting(oxr_engine* eh, oxr_launch_stats* st) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e) return fail(OXR_EINVAL, "null engine");
+    std::lock_guard<std::mutex> lk(e->mu);
+    int rc = check_sources(*e, e->num_photon, e->slot_offset);
+    if (rc) return rc;
+    rc = run_launch(*e, e->ev, e->stream, e->num_photon, int64_t(e->slot_offset), e->event_index, OXR_FORM_PLAIN, true, st,
+                    e->have_pre ? e->ev.pre.p : nullptr, e->have_pre ? e->npre : 0,
+                    e->have_input ? e->ev.input.p : nullptr);
+    if (rc == OXR_OK) e->launched = true;
+    return rc;
+}
+
+int oxr_event_get(oxr_engine* eh, int32_t what, void* dst, int64_t cap, int64_t* count) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !count) return fail(OXR_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
+    const void* src = nullptr;
+    int64_t n = 0;
+    size_t item = 0;
+    switch (what) {
+        case OXR_GET_PHOTON: src = e->ev.photon.p; n = e->num_photon; item = 64; break;
+        case OXR_GET_HIT: src = e->ev.hit.p; n = e->num_hit; item = 64; break;
+        case OXR_GET_SEQ:
+            if (!e->cfg.seq_enabled) return fail(OXR_ESTATE, "seq_enabled is off");
+            src = e->ev.seq.p; n = e->num_photon; item = 32; break;
+        case OXR_GET_RECORD:
+            if (e->cfg.max_record <= 0) return fail(OXR_ESTATE, "max_record is 0");
+            src = e->ev.record.p; n = e->num_photon * e->cfg.max_record; item = 64; break;
+        case OXR_GET_SEED: src = e->ev.seed.p; n = e->num_photon; item = 4; break;
+        case OXR_GET_GENSTEP: src = e->ev.gs.p; n = e->num_gs; item = 96; break;
+        default: return fail(OXR_EINVAL, "unknown gather kind");
+    }
+    if (what != OXR_GET_SEED && what != OXR_GET_GENSTEP && !e->launched) return fail(OXR_ESTATE, "no launch for this event");
+    *count = n;
+    int64_t m = std::min(n, std::max<int64_t>(cap, 0));
+    if (m > 0) {
+        if (!dst) return fail(OXR_EINVAL, "null destination");
+        OXR_HIP(hipMemcpy(dst, src, size_t(m) * item, hipMemcpyDeviceToHost));
+    }
+    return m < n ? fail(OXR_ECAPACITY, "destination smaller than count") : OXR_OK;
+}
+
+int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_index, int32_t form, void* hits,
+                 int64_t hit_cap, int64_t* num_hit, oxr_launch_stats* st) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !num_hit || num_gs < 0 || (num_gs > 0 && !gs)) return fail(OXR_EINVAL, "bad argument");
+    if (form != OXR_FORM_PLAIN && form != OXR_FORM_PERSISTENT) return fail(OXR_EINVAL, "unknown form");
+    oxr_launch_stats acc{};
+    int64_t total_hit = 0;
+    uint64_t slot = 0;
+    int64_t g0 = 0;
+    bool overflow = false;
+    while (g0 < num_gs || (num_gs == 0 && g0 == 0)) {
+        // genstep slice bounded by max_photon (SGenstep::GetGenstepSlices role)
+        int64_t g1 = g0, np = 0;
+        while (g1 < num_gs) {
+            int64_t k = int64_t(fbits(gs[24 * g1 + 3]));
+            if (k > e->cfg.max_photon) return fail(OXR_EINVAL, "single genstep exceeds max_photon");
+            if (np + k > e->cfg.max_photon) break;
+            np += k; g1++;
+        }
+        int64_t n = 0;
+        int rc = oxr_event_set_gensteps(eh, gs + 24 * g0, g1 - g0, event_index, slot, &n);
+        if (rc) return rc;
+        oxr_launch_stats s{};
+        rc = oxr_simulate_launch(eh, form, &s);
+        if (rc) return rc;
+        int64_t room = std::max<int64_t>(0, hit_cap - total_hit);
+        int64_t take = std::min(room, s.num_hit);
+        if (take > 0 && hits) {
+            std::lock_guard<std::mutex> lk(e->mu);
+            OXR_HIP(hipMemcpy(static_cast<char*>(hits) + size_t(total_hit) * 64, e->ev.hit.p, size_t(take) * 64, hipMemcpyDeviceToHost));
+        }
+        if (take < s.num_hit) overflow = true;
+        total_hit += s.num_hit;
+        acc.num_photon += s.num_photon; acc.num_hit += s.num_hit; acc.bounce_steps += s.bounce_steps;
+        acc.kernel_ms += s.kernel_ms; acc.compact_ms += s.compact_ms; acc.total_ms += s.total_ms;
+        slot += uint64_t(n);
+        if (num_gs == 0) break;
+        g0 = g1;
+    }
+    *num_hit = total_hit;
+    if (st) *st = acc;
+    return overflow ? fail(OXR_ECAPACITY, "hit buffer smaller than hit count") : OXR_OK;
+}
+
+int oxr_trace_rays(oxr_engine* eh, const float* rays, int64_t n, float* prd) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || n < 0 || (n > 0 && (!rays || !prd))) return fail(OXR_EINVAL, "bad argument");
+    if (n == 0) return OXR_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    DBuf<float4> dr, dp;
+    auto cleanup = [&]() { dr.release(); dp.release(); };
+    hipError_t x;
+    if ((x = dr.reserve(size_t(n) * 2)) != hipSuccess) { cleanup(); return hip_fail(x, "hipMalloc rays"); }
+    if ((x = dp.reserve(size_t(n) * 2)) != hipSuccess) { cleanup(); return hip_fail(x, "hipMalloc prd"); }
+    if ((x = hipMemcpy(dr.p, rays, size_t(n) * 32, hipMemcpyHostToDevice)) != hipSuccess) { cleanup(); return hip_fail(x, "copy rays"); }
+    EventBuffers dummy;
+    oxd::Params P = e->params(dummy, 0, 0, 0, e->ntop_for(256));
+    if ((x = oxl::trace_rays(e->stream, P, dr.p, n, dp.p, e->cfg.fp64_roots != 0)) != hipSuccess) { cleanup(); return hip_fail(x, "trace launch"); }
+    if ((x = hipStreamSynchronize(e->stream)) != hipSuccess) { cleanup(); return hip_fail(x, "trace sync"); }
+    x = hipMemcpy(prd, dp.p, size_t(n) * 32, hipMemcpyDeviceToHost);
+    cleanup();
+    if (x != hipSuccess) return hip_fail(x, "copy prd");
+    return OXR_OK;
+}
+
+int oxr_intersect_prim(oxr_engine* eh, int32_t prim_index, const float* rays, int64_t n, float* isect, int32_t* valid) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || n < 0 || (n > 0 && (!rays || !isect || !valid))) return fail(OXR_EINVAL, "bad argument");
+    if (prim_index < 0 || prim_index >= e->num_prim) return fail(OXR_EINVAL, "prim index out of range");
+    if (n == 0) return OXR_OK;
+    std::lock_guard<std::mutex> lk(e->mu);
+    int2 pr;
+    OXR_HIP(hipMemcpy(&pr, e->prim.p + prim_index, sizeof(int2), hipMemcpyDeviceToHost));
+    DBuf<float4> dr, di;
+    DBuf<int> dv;
+    auto cleanup = [&]() { dr.release(); di.release(); dv.release(); };
+    hipError_t x;
+    if ((x = dr.reserve(size_t(n) * 2)) != hipSuccess || (x = di.reserve(size_t(n))) != hipSuccess || (x = dv.reserve(size_t(n))) != hipSuccess) {
+        cleanup(); return hip_fail(x, "hipMalloc isect");
+    }
+    if ((x = hipMemcpy(dr.p, rays, size_t(n) * 32, hipMemcpyHostToDevice)) != hipSuccess) { cleanup(); return hip_fail(x, "copy rays"); }
+    if ((x = oxl::intersect_prim(e->stream, e->geom(), pr.x, dr.p, n, di.p, dv.p, e->cfg.fp64_roots != 0)) != hipSuccess) { cleanup(); return hip_fail(x, "isect launch"); }
+    if ((x = hipStreamSynchronize(e->stream)) != hipSuccess) { cleanup(); return hip_fail(x, "isect sync"); }
+    if ((x = hipMemcpy(isect, di.p, size_t(n) * 16, hipMemcpyDeviceToHost)) != hipSuccess) { cleanup(); return hip_fail(x, "copy isect"); }
+    x = hipMemcpy(valid, dv.p, size_t(n) * 4, hipMemcpyDeviceToHost);
+    cleanup();
+    if (x != hipSuccess) return hip_fail(x, "copy valid");
+    return OXR_OK;
+}
+
+int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_results, int32_t* num_results) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || repeat < 1) return fail(OXR_EINVAL, "bad argument");
+    if (e->num_photon <= 0) return fail(OXR_ESTATE, "autotune needs an event with photons");
+    oxr_tuning base = e->tune, best = e->tune;
+    double best_ms = 1e300;
+    int nres = 0;
+    auto measure = [&](const oxr_tuning& t, double& ms) -> int {
+        int rc = oxr_engine_set_tuning(eh, &t);
+        if (rc) return rc;
+        double tot = 0.0;
+        for (int r = 0; r < repeat; r++) {
+            oxr_launch_stats s{};
+            rc = oxr_simulate_launch(eh, OXR_FORM_PERSISTENT, &s);
+            if (rc) return rc;
+            tot += s.kernel_ms;
+        }
+        ms = tot / repeat;
+        if (results && nres < max_results) {
+            double* r = results + 5 * nres;
+            r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch; r[4] = ms;
+        }
+        nres++;
+        if (ms < best_ms) { best_ms = ms; best = t; }
+        return OXR_OK;
+    };
+    // coordinate search: launch shape, then LDS staging, then refill granularity
+    for (int bs : {64, 128, 256}) {
+        for (int bpc : {1, 2, 3, 4, 6, 8}) {
+            oxr_tuning t = base; t.block_size = bs; t.blocks_per_cu = bpc;
+            if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, 0), bs) > 64 * 1024) continue;
+            double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+        }
+    }
+    oxr_tuning b1 = best;
+    for (int nt : {0, 16, 64, 256, 512}) {
+        oxr_tuning t = b1; t.lds_top_nodes = nt;
+        double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+    }
+    oxr_tuning b2 = best;
+    for (int rb : {1, 8, 16, 32, 48, 64}) {
+        oxr_tuning t = b2; t.refill_batch = rb;
+        double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+    }
+    oxr_engine_set_tuning(eh, &best);
+    if (num_results) *num_results = nres;
+    return OXR_OK;
+}
+
+} // extern "C"

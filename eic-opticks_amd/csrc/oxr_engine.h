@@ -1,0 +1,104 @@
+// oxr_engine.h -- host-side engine state shared by the C-ABI (oxr_engine.cpp) and the
+// serving loop (oxr_server.cpp). The engine owns every device allocation it makes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <mutex>
+#include "../../include/oxr.h"
+#include "oxr_kernels.h"
+
+namespace oxh {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define OXR_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t _e = (call);                                              \
+        if (_e != hipSuccess) return ::oxh::hip_fail(_e, #call);             \
+    } while (0)
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;     // capacity in elements
+    hipError_t reserve(size_t want) {
+        if (want <= n) return hipSuccess;
+        if (p) { hipError_t e = hipFree(p); if (e != hipSuccess) return e; p = nullptr; n = 0; }
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), want * sizeof(T));
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+// One event slice's device buffers (the QEvt role).
+struct EventBuffers {
+    DBuf<float4> gs;           // 6 float4 per genstep
+    DBuf<long long> gs_off;    // num_gs + 1 prefix offsets
+    DBuf<int> seed;
+    DBuf<float4> photon;       // 4 float4 per photon
+    DBuf<ulonglong2> seq;      // 2 per photon
+    DBuf<float4> record;
+    DBuf<uint8_t> hitflag;
+    DBuf<float4> hit;
+    DBuf<int> counts;
+    DBuf<long long> offs;
+    DBuf<unsigned long long> counters;
+    DBuf<float4> input;
+    DBuf<float> pre;
+    void release_all();
+};
+
+struct Engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    std::string name, arch;
+    oxr_config cfg{};
+    oxr_tuning tune{};
+    // geometry
+    DBuf<float4> node, plan, itra, bnd;
+    DBuf<int2> prim;
+    DBuf<oxd::BNode> tlas, blas;
+    DBuf<oxd::InstD> inst;
+    DBuf<uint4> optical;
+    DBuf<float> icdf;
+    int tlas_root = 0;
+    int num_inst = 0, num_solid = 0, num_prim = 0, num_node = 0;
+    int64_t tlas_nodes = 0, blas_nodes = 0;
+    int stack_need = 0;
+    std::vector<int> solid_prim_offset, solid_num_prim, solid_blas_root;
+    oxd::Tables tables{};
+    int64_t geometry_bytes = 0;
+    // current event (single-slice state, as QEvt)
+    EventBuffers ev;
+    std::vector<float> h_gs;
+    int64_t num_gs = 0, num_photon = 0, num_hit = 0;
+    int32_t event_index = 0;
+    uint64_t slot_offset = 0;
+    bool have_input = false, have_pre = false;
+    int64_t input_n = 0;
+    int32_t npre = 0;
+    int64_t pre_rows = 0;
+    int num_bnd = 0;
+    bool launched = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    std::mutex mu;   // serialises C-ABI calls on one engine
+
+    oxd::Geom geom() const;
+    oxd::Params params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int ntop) const;
+    int stack_cap() const;
+    int ntop_for(int block) const;
+};
+
+// upload a gensteps slice into b, fill the seed array; returns photons in the slice
+int upload_gensteps(Engine& e, EventBuffers& b, hipStream_t s, const float* gs, int64_t num_gs, int64_t* num_photon);
+// run one simulate launch (+ compaction) on b for n photons; stats filled
+int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slot_off, int32_t evt_idx,
+               int form, bool count, oxr_launch_stats* st, const float* pre, int npre, const float4* input);
+int ensure_event_capacity(Engine& e, EventBuffers& b, int64_t n);
+
+} // namespace oxh

@@ -1,0 +1,70 @@
+// oxr_kernels.h -- kernel parameter block and launcher declarations shared by the
+// kernels (oxr_kernels.hip) and the host engine (oxr_engine.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "oxr_physics.h"
+
+namespace oxd {
+
+// One event's contiguous photon range inside a batched serving step. Step indices
+// [start, start+count) belong to this event; [start+count, next start) is padding so
+// that every segment begins on a compaction chunk.
+struct Seg {
+    long long start, count;
+    unsigned long long photon0;   // photon index within the event of the first photon
+    const float4* gs;
+    const int* seed;              // genstep index per photon of the whole event
+    int event_index;
+    int pad;
+};
+
+// Everything one simulate launch reads or writes (the role of CSGOptiX/Params.h:29-122
+// plus the device sevent, sysrap/sevent.h:105-233).
+struct Params {
+    Geom g;
+    Tables t;
+    const float4* gs;          // quad6 gensteps
+    const int* seed;           // genstep index per photon
+    const float4* input;       // input photons (INPUT_PHOTON gensteps)
+    const float* pre;          // precooked uniforms (num_photon, npre) or null
+    int npre;
+    float4* photon;            // sphoton out
+    ulonglong2* seq;           // sseq out (2 x ulonglong2 per photon) or null
+    float4* record;            // records or null
+    int max_record;
+    uint8_t* hitflag;          // 1 if flagmask & hitmask
+    uint32_t* bounce_out;      // optional per-photon loop count
+    uint32_t* overflow_out;    // optional precooked-stream wrap count
+    unsigned long long* counters;   // [0] steps [1] queue [2] node visits [3] prim tests [4] errors
+    long long num_photon;
+    unsigned long long slot_offset;
+    int event_index;
+    int max_bounce;
+    float tmin, tmin0;
+    unsigned eps0_mask;
+    float tmax, max_time;
+    unsigned hitmask;
+    unsigned long long rng_seed, rng_offset, skip;
+    int aligned;
+    int ntop;                  // top-level BVH nodes staged in LDS
+    int stack_cap;             // per-lane traversal stack entries in LDS
+    int refill_threshold;      // persistent form: idle lanes per wave before a refill
+    const Seg* segs;           // batched serving step, or null for a single event
+    int nseg;
+};
+
+} // namespace oxd
+
+namespace oxl {
+hipError_t fill_seed(hipStream_t s, const long long* offsets, int num_gs, int* seed, long long n);
+size_t simulate_smem(const oxd::Params& P, int block);
+hipError_t simulate_plain(hipStream_t s, const oxd::Params& P, int block, bool fp64, bool count);
+hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int grid, bool fp64);
+int compact_blocks(long long n);
+hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
+                   long long* offs, float4* hit, long long hit_cap);
+hipError_t trace_rays(hipStream_t s, const oxd::Params& P, const float4* rays, long long n, float4* prd, bool fp64);
+hipError_t intersect_prim(hipStream_t s, const oxd::Geom& g, int root, const float4* rays, long long n,
+                          float4* isect, int* valid, bool fp64);
+} // namespace oxl

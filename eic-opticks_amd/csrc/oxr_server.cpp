@@ -1,0 +1,327 @@
+// oxr_server.cpp -- serving form: a request loop that keeps geometry, tables and the RNG
+// definition resident and batches genstep requests from many events into shared device
+// steps (continuous batching). Requests are admitted as the HBM budget and the active
+// slot count allow; each step gives every active request a share of the step's photon
+// budget, shortest remaining first; one launch covers all of them through a segment
+// table; each request completes as soon as its last photon has run and its hits are
+// returned on their own. Photon random streams are keyed by (event_index, photon index
+// within the event), so a request's result does not depend on what it was batched with.
+#include "oxr_engine.h"
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+namespace oxh {
+
+int genstep_offsets(Engine& e, const float* gs, int64_t num_gs, std::vector<long long>& off, int64_t* n);
+
+struct Req {
+    int64_t id = 0;
+    int32_t event_index = 0;
+    std::vector<float> gs;
+    int64_t num_gs = 0, num_photon = 0;
+    std::vector<long long> off;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    int status = 0;
+    int64_t bytes = 0;
+    DBuf<float4> d_gs;
+    DBuf<int> d_seed;
+    DBuf<long long> d_off;
+    int64_t dispatched = 0;
+    std::vector<float> hits;
+    int64_t nhit = 0;
+    void free_device() { d_gs.release(); d_seed.release(); d_off.release(); }
+};
+
+struct Server {
+    Engine* e = nullptr;
+    oxr_server_config cfg{};
+    hipStream_t stream = nullptr;
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::unique_ptr<Req>> incoming;
+    std::vector<std::unique_ptr<Req>> active;
+    std::deque<int64_t> done_ids;
+    std::map<int64_t, std::unique_ptr<Req>> finished;
+    int64_t budget = 0, used = 0;
+    bool stop = false;
+    EventBuffers step;
+    DBuf<oxd::Seg> d_segs;
+    hipEvent_t e_ev0 = nullptr, e_ev1 = nullptr;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double stat_steps = 0, stat_photons = 0, stat_busy_ms = 0, stat_kernel_ms = 0, stat_bounce = 0, stat_hits = 0;
+    double clock() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+    void loop();
+    int admit_upload(Req& r);
+    int run_step();
+    void complete(std::unique_ptr<Req>& r, int status);
+};
+
+int Server::admit_upload(Req& r) {
+    size_t np = size_t(std::max<int64_t>(r.num_photon, 1));
+    OXR_HIP(r.d_gs.reserve(size_t(std::max<int64_t>(r.num_gs, 1)) * 6));
+    OXR_HIP(r.d_off.reserve(size_t(r.num_gs) + 1));
+    OXR_HIP(r.d_seed.reserve(np));
+    if (r.num_gs > 0) {
+        OXR_HIP(hipMemcpyAsync(r.d_gs.p, r.gs.data(), size_t(r.num_gs) * 96, hipMemcpyHostToDevice, stream));
+        OXR_HIP(hipMemcpyAsync(r.d_off.p, r.off.data(), (size_t(r.num_gs) + 1) * sizeof(long long), hipMemcpyHostToDevice, stream));
+        OXR_HIP(oxl::fill_seed(stream, r.d_off.p, int(r.num_gs), r.d_seed.p, r.num_photon));
+    }
+    return OXR_OK;
+}
+
+void Server::complete(std::unique_ptr<Req>& r, int status) {
+    r->status = status;
+    r->t_done = clock();
+    r->free_device();
+    std::lock_guard<std::mutex> lk(mu);
+    used -= r->bytes;
+    done_ids.push_back(r->id);
+    finished[r->id] = std::move(r);
+    cv_done.notify_all();
+}
+
+int Server::run_step() {
+    const long long CH = 4096;
+    std::vector<Req*> order;
+    for (auto& r : active) if (r->dispatched < r->num_photon) order.push_back(r.get());
+    std::sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
+        return (a->num_photon - a->dispatched) < (b->num_photon - b->dispatched);
+    });
+    std::vector<oxd::Seg> segs;
+    std::vector<Req*> owner;
+    std::vector<int64_t> take;
+    long long cursor = 0;
+    int64_t left = std::max<int64_t>(cfg.max_step_photons, CH);
+    for (Req* r : order) {
+        if (left <= 0) break;
+        int64_t t = std::min(left, r->num_photon - r->dispatched);
+        oxd::Seg s{};
+        s.start = cursor; s.count = t;
+        s.photon0 = uint64_t(r->dispatched);
+        s.gs = r->d_gs.p; s.seed = r->d_seed.p;
+        s.event_index = r->event_index;
+        segs.push_back(s); owner.push_back(r); take.push_back(t);
+        cursor += ((t + CH - 1) / CH) * CH;
+        left -= t;
+    }
+    if (segs.empty()) return OXR_OK;
+    auto w0 = std::chrono::steady_clock::now();
+    int64_t N = cursor;
+    int rc = ensure_event_capacity(*e, step, N);
+    if (rc) return rc;
+    OXR_HIP(d_segs.reserve(segs.size()));
+    OXR_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(oxd::Seg), hipMemcpyHostToDevice, stream));
+    int form = cfg.form;
+    int block = form == OXR_FORM_PERSISTENT ? e->tune.block_size : 256;
+    oxd::Params P = e->params(step, N, 0, 0, e->ntop_for(block));
+    P.seq = nullptr; P.record = nullptr;
+    P.segs = d_segs.p; P.nseg = int(segs.size());
+    OXR_HIP(hipMemsetAsync(step.counters.p, 0, 8 * sizeof(unsigned long long), stream));
+    OXR_HIP(hipEventRecord(e_ev0, stream));
+    if (form == OXR_FORM_PERSISTENT) {
+        int grid = std::max(1, e->tune.blocks_per_cu) * std::max(1, e->num_cu);
+        OXR_HIP(oxl::simulate_persist(stream, P, block, grid, e->cfg.fp64_roots != 0));
+    } else {
+        OXR_HIP(oxl::simulate_plain(stream, P, block, e->cfg.fp64_roots != 0, false));
+    }
+    OXR_HIP(hipEventRecord(e_ev1, stream));
+    OXR_HIP(oxl::compact(stream, step.hitflag.p, step.photon.p, N, step.counts.p, step.offs.p, step.hit.p, N));
+    int nb = oxl::compact_blocks(N);
+    std::vector<long long> offs(size_t(nb) + 1);
+    unsigned long long ctr[8];
+    OXR_HIP(hipMemcpyAsync(offs.data(), step.offs.p, offs.size() * sizeof(long long), hipMemcpyDeviceToHost, stream));
+    OXR_HIP(hipMemcpyAsync(ctr, step.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, stream));
+    OXR_HIP(hipStreamSynchronize(stream));
+    if (ctr[4] != 0) return fail(OXR_EINVAL, "traversal stack overflow");
+    long long nh = offs[size_t(nb)];
+    std::vector<float> hbuf(size_t(nh) * 16);
+    if (nh > 0) OXR_HIP(hipMemcpy(hbuf.data(), step.hit.p, size_t(nh) * 64, hipMemcpyDeviceToHost));
+    float kms = 0.f;
+    (void)hipEventElapsedTime(&kms, e_ev0, e_ev1);
+    for (size_t k = 0; k < segs.size(); k++) {
+        long long h0 = offs[size_t(segs[k].start / CH)];
+        long long h1 = k + 1 < segs.size() ? offs[size_t(segs[k + 1].start / CH)] : nh;
+        Req* r = owner[k];
+        r->hits.insert(r->hits.end(), hbuf.begin() + h0 * 16, hbuf.begin() + h1 * 16);
+        r->nhit += h1 - h0;
+        r->dispatched += take[k];
+    }
+    auto w1 = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(mu);
+    stat_steps += 1;
+    for (auto t : take) stat_photons += double(t);
+    stat_busy_ms += std::chrono::duration<double, std::milli>(w1 - w0).count();
+    stat_kernel_ms += kms;
+    stat_bounce += double(ctr[0]);
+    stat_hits += double(nh);
+    return OXR_OK;
+}
+
+void Server::loop() {
+    for (;;) {
+        std::vector<std::unique_ptr<Req>> admitted;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !incoming.empty() || !active.empty(); });
+            if (stop) break;
+            while (!incoming.empty()) {
+                Req& r = *incoming.front();
+                bool slot = int(active.size() + admitted.size()) < cfg.max_active;
+                bool mem = used + r.bytes <= budget || (active.empty() && admitted.empty());
+                if (!slot || !mem) break;
+                used += r.bytes;
+                r.t_admit = clock();
+                admitted.push_back(std::move(incoming.front()));
+                incoming.pop_front();
+            }
+        }
+        for (auto& r : admitted) {
+            int rc = admit_upload(*r);
+            if (rc != OXR_OK || r->num_photon == 0) complete(r, rc);
+            else active.push_back(std::move(r));
+        }
+        if (active.empty()) continue;
+        int rc = run_step();
+        if (rc != OXR_OK) {
+            for (auto& r : active) complete(r, rc);
+            active.clear();
+            continue;
+        }
+        for (auto& r : active) if (r->dispatched >= r->num_photon) complete(r, OXR_OK);
+        active.erase(std::remove_if(active.begin(), active.end(), [](const std::unique_ptr<Req>& r) { return !r; }), active.end());
+    }
+    for (auto& r : active) complete(r, OXR_ESTATE);
+    active.clear();
+}
+
+} // namespace oxh
+
+using namespace oxh;
+
+extern "C" {
+
+int oxr_server_create(oxr_engine* eh, const oxr_server_config* cfg, oxr_server** out) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !out) return fail(OXR_EINVAL, "null argument");
+    *out = nullptr;
+    Server* s = new Server();
+    s->e = e;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.max_step_photons <= 0) s->cfg.max_step_photons = 4 << 20;
+    if (s->cfg.hit_fraction <= 0.f) s->cfg.hit_fraction = 0.25f;
+    if (s->cfg.form != OXR_FORM_PLAIN) s->cfg.form = OXR_FORM_PERSISTENT;
+    if (s->cfg.max_active <= 0) s->cfg.max_active = 256;
+    hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) { delete s; return hip_fail(he, "server stream"); }
+    if ((he = hipEventCreate(&s->e_ev0)) != hipSuccess || (he = hipEventCreate(&s->e_ev1)) != hipSuccess) {
+        (void)hipStreamDestroy(s->stream); delete s; return hip_fail(he, "server events");
+    }
+    int rc = ensure_event_capacity(*e, s->step, s->cfg.max_step_photons + 4096 * 64);
+    if (rc) { s->step.release_all(); (void)hipStreamDestroy(s->stream); delete s; return rc; }
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    s->budget = s->cfg.hbm_budget_bytes > 0 ? s->cfg.hbm_budget_bytes : int64_t(0.8 * double(fr));
+    s->th = std::thread([s] { s->loop(); });
+    *out = reinterpret_cast<oxr_server*>(s);
+    return OXR_OK;
+}
+
+void oxr_server_destroy(oxr_server* sh) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    (void)hipStreamSynchronize(s->stream);
+    for (auto& r : s->incoming) r->free_device();
+    for (auto& kv : s->finished) kv.second->free_device();
+    s->step.release_all();
+    s->d_segs.release();
+    (void)hipEventDestroy(s->e_ev0);
+    (void)hipEventDestroy(s->e_ev1);
+    (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+double oxr_server_clock(oxr_server* sh) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    return s ? s->clock() : 0.0;
+}
+
+int oxr_server_submit(oxr_server* sh, int64_t id, const float* gs, int64_t num_gs, int32_t event_index) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || num_gs < 0 || (num_gs > 0 && !gs)) return fail(OXR_EINVAL, "bad argument");
+    auto r = std::make_unique<Req>();
+    r->id = id;
+    r->event_index = event_index;
+    r->gs.assign(gs, gs + 24 * num_gs);
+    r->num_gs = num_gs;
+    r->t_submit = s->clock();
+    int64_t n = 0;
+    int rc = genstep_offsets(*s->e, r->gs.data(), num_gs, r->off, &n);
+    if (rc) return rc;
+    r->num_photon = n;
+    r->bytes = 96 * num_gs + 8 * (num_gs + 1) + n * (4 + 64) + int64_t(double(s->cfg.hit_fraction) * 64.0 * double(n));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->finished.count(id)) return fail(OXR_EINVAL, "request id already completed and not taken");
+        s->incoming.push_back(std::move(r));
+    }
+    s->cv_work.notify_all();
+    return OXR_OK;
+}
+
+int oxr_server_wait(oxr_server* sh, oxr_request_result* res, int32_t timeout_ms) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !res) return fail(OXR_EINVAL, "bad argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return !s->done_ids.empty(); };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) return fail(OXR_ETIMEOUT, "no request completed");
+    int64_t id = s->done_ids.front();
+    s->done_ids.pop_front();
+    Req& r = *s->finished[id];
+    res->request_id = r.id; res->event_index = r.event_index; res->status = r.status;
+    res->num_photon = r.num_photon; res->num_hit = r.nhit; res->bounce_steps = 0;
+    res->t_submit = r.t_submit; res->t_admit = r.t_admit; res->t_done = r.t_done;
+    return OXR_OK;
+}
+
+int oxr_server_take_hits(oxr_server* sh, int64_t id, void* dst, int64_t cap, int64_t* count) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !count) return fail(OXR_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->finished.find(id);
+    if (it == s->finished.end()) return fail(OXR_ENOTFOUND, "unknown or not yet completed request");
+    Req& r = *it->second;
+    *count = r.nhit;
+    int64_t m = std::min(r.nhit, std::max<int64_t>(cap, 0));
+    if (m > 0) {
+        if (!dst) return fail(OXR_EINVAL, "null destination");
+        std::memcpy(dst, r.hits.data(), size_t(m) * 64);
+    }
+    if (m < r.nhit) return fail(OXR_ECAPACITY, "destination smaller than hit count");
+    s->finished.erase(it);
+    return OXR_OK;
+}
+
+int oxr_server_stats(oxr_server* sh, double* o) {
+    Server* s = reinterpret_cast<Server*>(sh);
+    if (!s || !o) return fail(OXR_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    o[0] = s->stat_steps; o[1] = s->stat_photons; o[2] = s->stat_busy_ms; o[3] = s->stat_kernel_ms;
+    o[4] = s->stat_bounce; o[5] = s->stat_hits; o[6] = double(s->used); o[7] = double(s->budget);
+    return OXR_OK;
+}
+
+} // extern "C"
