@@ -1,0 +1,304 @@
+"""Benchmark: photons/s and bounce-steps/s on one MI355X.
+
+Headline step: one simulate launch (generate + propagate + hit compaction) of the
+crystal-calorimeter workload (8x8 CsI crystals with SiPMs, 2000 scintillation + 500
+Cerenkov gensteps, ~2.08M photons) with gensteps resident in HBM, persistent form with
+the committed autotuned launch settings. Also reported: the plain form, the roofline of
+the simulate kernel, a CPU baseline (the CPU restatement on the host cores), the serving
+form's p50/p99 event latency against sequential per-event launches, and served vs plain
+photons/s on the instanced-CSG geometry.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(ROOT, 'eic-opticks_amd'), os.path.join(ROOT, 'oracle')]
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ensure_built():
+    from oxr import engine
+    if not os.path.exists(engine.LIB_PATH):
+        from oxr import build
+        build.build()
+
+
+def load_tuning(name):
+    path = os.path.join(ROOT, 'eic-opticks_amd', 'oxr', 'tuning_gfx950.json')
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get(name)
+    return None
+
+
+def read_traffic(workload):
+    """HBM bytes per launch of the simulate kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+    path = os.path.join(ROOT, 'profiles', 'pmc_traffic.txt')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == workload:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def algorithmic_bytes(stats, n_photon, n_gs, n_hit):
+    """Bytes the hot path must move per launch, counted at visit granularity by the
+    counting launch: per photon 4 (seed) + 96/ppg (genstep) + 64 (photon) + 1 (hit flag);
+    per BVH node visit 64; per instance entered 64; per prim test 64 per CSG node plus 64
+    per transformed leaf; per bounce step 4 table lookups x 2 texels x 16 B + 16 B optical;
+    per hit 64 read + 64 write (compaction)."""
+    ppg = n_photon / max(n_gs, 1)
+    per_photon = 4 + 96.0 / ppg + 64 + 1
+    per_step = 4 * 2 * 16 + 16
+    return (n_photon * per_photon + stats['node_visits'] * 64 + stats['instance_visits'] * 64
+            + stats['csg_node_bytes'] + stats['bounce_steps'] * per_step + n_hit * 128)
+
+
+def timed_launches(e, form, steps, warmup):
+    from oxr import abi
+    for _ in range(warmup):
+        e.launch(form)
+    ks = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = e.launch(form)
+        ks.append(st['kernel_ms'])
+    dt = time.perf_counter() - t0
+    return dt, st, float(np.mean(ks))
+
+
+def cpu_baseline(scene, gs, cfg, budget_s=12.0):
+    """The CPU restatement (oracle) of the same workload on the host cores: a bounded
+    sample of the event's photons, photons/s over the sample."""
+    import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    o = O.Oracle(scene.geometry, scene.physics, cfg)
+    o.set_cull(True)
+    n_total = int(np.asarray(gs).view(np.uint32)[:, 0, 3].astype(np.int64).sum())
+    probe = min(4000, n_total)
+    t0 = time.perf_counter()
+    o.simulate(gs, nthreads=threads, begin=0, end=probe, seq=False)
+    rate = probe / max(time.perf_counter() - t0, 1e-6)
+    sample = int(min(n_total, max(probe, rate * budget_s)))
+    t0 = time.perf_counter()
+    r = o.simulate(gs, nthreads=threads, begin=0, end=sample, seq=False)
+    dt = time.perf_counter() - t0
+    return dict(value=sample / dt, unit='photons/s', cores=threads, kind='port',
+                sample=f'first {sample} of {n_total} photons of the same event, {threads} std::threads, '
+                       f'{dt:.1f} s', bounce_steps_per_s=float(r['bounce'].sum()) / dt)
+
+
+def serving_events(rng, n_events, median, sigma, lo, hi, ext):
+    counts = np.clip(np.exp(np.log(median) + sigma * rng.normal(size=n_events)), lo, hi).astype(np.int64)
+    from oxr import gensteps as G
+    evs = []
+    for c in counts:
+        ngs = int(max(1, min(64, c // 20000)))
+        pos = rng.uniform(-ext, ext, size=(ngs, 3))
+        per = np.full(ngs, c // ngs)
+        per[0] += c - per.sum()
+        evs.append(G.torch(per, pos=pos, wavelength=420.0, type=G.T_POINT_ISOTROPIC))
+    return evs, counts
+
+
+def run_server(engine_mod, e, evs, arrivals, step_photons):
+    srv = engine_mod.Server(e, max_step_photons=step_photons)
+    res = {}
+
+    def feeder():
+        t0 = time.perf_counter()
+        for k, (gs, t) in enumerate(zip(evs, arrivals)):
+            d = t - (time.perf_counter() - t0)
+            if d > 0:
+                time.sleep(d)
+            srv.submit(k, gs, 1000 + k)
+
+    th = threading.Thread(target=feeder)
+    t_start = time.perf_counter()
+    th.start()
+    photons = 0
+    for _ in evs:
+        r = srv.wait(600000)
+        srv.take_hits(r['request_id'], r['num_hit'])
+        res[r['request_id']] = r['t_done'] - r['t_submit']
+        photons += r['num_photon']
+    wall = time.perf_counter() - t_start
+    th.join()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([res[k] for k in range(len(evs))])
+    return lat, wall, photons, st
+
+
+def run_sequential(e, evs, arrivals):
+    """Per-event plain launches, one event at a time in arrival order (the reference's
+    one-request-at-a-time service, CSGOptiXService.h:76-84)."""
+    from oxr import abi
+    lat = np.zeros(len(evs))
+    t0 = time.perf_counter()
+    busy = 0.0
+    photons = 0
+    for k, (gs, t) in enumerate(zip(evs, arrivals)):
+        now = time.perf_counter() - t0
+        if now < t:
+            time.sleep(t - now)
+        a = time.perf_counter()
+        h, st = e.simulate(gs, event_index=1000 + k, form=abi.OXR_FORM_PLAIN)
+        b = time.perf_counter()
+        busy += b - a
+        photons += st['num_photon']
+        lat[k] = (b - t0) - t
+    return lat, time.perf_counter() - t0, photons, busy
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=2)
+    ap.add_argument('--quick', action='store_true', help='skip serving, CPU baseline and instanced-CSG legs')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-serving', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one MI355X: this engine runs one process on one GPU and the '
+              'multi-GPU form is a design note only; run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+    ensure_built()
+    from oxr import abi, scenes, gensteps as G
+    from oxr import engine as engine_mod
+
+    # ---------------------------------------------------------------- headline
+    name = 'csi_calorimeter'
+    s = scenes.csi_calorimeter()
+    gs = s.gensteps()
+    cfg = abi.default_config(**s.config)
+    e = engine_mod.Engine(s.geometry, s.physics, cfg)
+    info = e.info()
+    tune = load_tuning(name)
+    if tune:
+        e.set_tuning(**tune)
+    n = e.set_gensteps(gs)
+    log(f'[bench] {info["device_name"]} {info["arch"]} CUs={info["num_cu"]} workload={name} photons={n} tuning={e.tuning()}')
+    dt, st, kms = timed_launches(e, abi.OXR_FORM_PERSISTENT, args.steps, args.warmup)
+    value = args.steps * n / dt
+    steps_per_s = args.steps * st['bounce_steps'] / dt
+    log(f'[bench] persistent: {value / 1e6:.1f} M photons/s, {steps_per_s / 1e6:.1f} M steps/s, kernel {kms:.2f} ms')
+    dtp, stp, kmsp = timed_launches(e, abi.OXR_FORM_PLAIN, args.steps, args.warmup)
+    plain_value = args.steps * n / dtp
+    log(f'[bench] plain:      {plain_value / 1e6:.1f} M photons/s, kernel {kmsp:.2f} ms')
+    cnt = e.launch(counting=True)
+    ab = algorithmic_bytes(cnt, n, len(gs), st['num_hit'])
+    achieved = ab / (kms / 1e3) / 1e9
+    traffic = read_traffic(name)
+    roof = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel='k_simulate_persist', algorithmic_bytes_per_launch=int(ab),
+                node_visits=int(cnt['node_visits']), prim_tests=int(cnt['prim_tests']),
+                csg_node_bytes=int(cnt['csg_node_bytes']), instance_visits=int(cnt['instance_visits']))
+    log(f'[bench] roofline: {achieved:.0f} GB/s algorithmic ({ab / 1e9:.2f} GB/launch)')
+    out = {
+        'metric': 'photons/s & bounce-steps/s on 1 MI355X; p99 event latency vs per-event launch',
+        'value': round(value, 1), 'unit': 'photons/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(1e3 * dt / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': 'csi_calorimeter: 8x8 CsI 20x20x200 mm crystals + 6x6 mm SiPMs, '
+                               '2000 scintillation + 500 Cerenkov gensteps, Poisson 1000/150 photons',
+                   'photons': n, 'gensteps': len(gs), 'form': 'persistent', 'tuning': e.tuning(),
+                   'max_bounce': cfg.max_bounce},
+        'bounce_steps_per_s': round(steps_per_s, 1),
+        'kernel_ms': round(kms, 3),
+        'plain_form': {'photons_per_s': round(plain_value, 1), 'kernel_ms': round(kmsp, 3)},
+        'roofline': roof,
+    }
+    e.set_gensteps(gs)
+    if not (args.quick or args.no_cpu):
+        cb = cpu_baseline(s, gs, cfg)
+        out['cpu_baseline'] = cb
+        log(f"[bench] cpu baseline: {cb['value']:.0f} photons/s on {cb['cores']} threads")
+    else:
+        out['cpu_baseline'] = None
+
+    if not (args.quick or args.no_serving):
+        # ------------------------------------------------------------ serving (C2 geometry)
+        rng = np.random.default_rng(42)
+        evs, counts = serving_events(rng, 256, 200_000, 1.0, 1_000, 20_000_000, 60.0)
+        arrivals = np.cumsum(rng.exponential(1 / 40.0, size=256))
+        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e, evs, arrivals, 4 << 20)
+        lat_b, wall_b, ph_b, busy_b = run_sequential(e, evs, arrivals)
+        out['serving'] = {
+            'workload': '256 events, Poisson arrivals 40/s, log-normal photons (median 200k, sigma 1, '
+                        'clipped 1k-20M), torch gensteps, csi_calorimeter geometry resident',
+            'photons': int(counts.sum()),
+            'served': {'p50_ms': round(1e3 * float(np.percentile(lat_s, 50)), 2),
+                       'p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 2),
+                       'photons_per_s_wall': round(ph_s / wall_s, 1),
+                       'photons_per_s_busy': round(sst['photons'] / (sst['busy_ms'] / 1e3), 1),
+                       'steps': int(sst['steps'])},
+            'per_event_plain': {'p50_ms': round(1e3 * float(np.percentile(lat_b, 50)), 2),
+                                'p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 2),
+                                'photons_per_s_wall': round(ph_b / wall_b, 1),
+                                'photons_per_s_busy': round(ph_b / busy_b, 1)},
+        }
+        log(f"[bench] serving: {out['serving']}")
+    e.close()
+
+    if not args.quick:
+        # ------------------------------------------------------------ instanced CSG
+        s3 = scenes.instanced_csg()
+        cfg3 = abi.default_config(**s3.config)
+        e3 = engine_mod.Engine(s3.geometry, s3.physics, cfg3)
+        t3 = load_tuning('instanced_csg')
+        if t3:
+            e3.set_tuning(**t3)
+        gs3 = s3.gensteps()
+        n3 = e3.set_gensteps(gs3)
+        dt3, st3, k3 = timed_launches(e3, abi.OXR_FORM_PERSISTENT, 3, 1)
+        dt3p, st3p, k3p = timed_launches(e3, abi.OXR_FORM_PLAIN, 3, 1)
+        rng = np.random.default_rng(43)
+        evs, counts = serving_events(rng, 128, 200_000, 1.0, 1_000, 20_000_000, 16000.0)
+        zero = np.zeros(len(evs))
+        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e3, evs, zero, 8 << 20)
+        lat_b, wall_b, ph_b, busy_b = run_sequential(e3, evs, zero)
+        out['instanced_csg'] = {
+            'workload': 'water sphere r=19.5 m, acrylic shell, 20000 instanced 7-node CSG solids, 10M torch photons',
+            'photons': n3,
+            'persistent_photons_per_s': round(3 * n3 / dt3, 1), 'persistent_kernel_ms': round(k3, 2),
+            'plain_photons_per_s': round(3 * n3 / dt3p, 1), 'plain_kernel_ms': round(k3p, 2),
+            'bounce_steps': int(st3['bounce_steps']),
+            'served_vs_per_event': {
+                'events': len(evs), 'photons': int(counts.sum()),
+                'served_photons_per_s': round(ph_s / wall_s, 1),
+                'per_event_plain_photons_per_s': round(ph_b / wall_b, 1),
+                'ratio': round((ph_s / wall_s) / (ph_b / wall_b), 3),
+                'served_p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 1),
+                'per_event_p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 1)},
+        }
+        log(f"[bench] instanced_csg: {out['instanced_csg']}")
+        e3.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -46,18 +46,21 @@ enum { CSG_ZERO = 0, CSG_NODE = 11, CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHER
 // ------------------------------------------------------------------ Philox4x32-10
 // Counter-based generator of Salmon et al. (SC'11). Stream of photon p: key = seed,
 // counter = (block lo, block hi, p lo, p hi), uniform j at block (start+j)/4, lane (start+j)%4.
+// The four outputs of the current block are held in scalars b0..b3 and consumed by
+// shifting, so the state never needs an indexed (memory-resident) array.
 struct Rng {
     uint32_t k0, k1;
     uint32_t p0, p1;          // photon index (subsequence)
-    uint64_t pos;             // absolute uniform index
-    uint32_t b[4];
-    uint64_t blk;
+    uint64_t pos;             // absolute uniform index of the next uniform
+    uint32_t b0, b1, b2, b3;  // unconsumed outputs of the current block, b0 next
+    int have;                 // how many of b0..b3 are valid
     const float* pre;         // precooked row or null
     int32_t npre, cur;
     uint32_t overflow;
 };
 
-OXD void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1, uint32_t* o) {
+OXD void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
@@ -66,14 +69,15 @@ OXD void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
-    o[0] = c0; o[1] = c1; o[2] = c2; o[3] = c3;
+    o0 = c0; o1 = c1; o2 = c2; o3 = c3;
 }
 
 OXD void rng_init(Rng& r, uint64_t seed, uint64_t offset, uint64_t skip, uint64_t event, uint64_t photon) {
     r.k0 = uint32_t(seed); r.k1 = uint32_t(seed >> 32);
     r.p0 = uint32_t(photon); r.p1 = uint32_t(photon >> 32);
     r.pos = offset + skip * event;
-    r.blk = ~0ull;
+    r.have = 0;
+    r.b0 = r.b1 = r.b2 = r.b3 = 0u;
     r.pre = nullptr; r.npre = 0; r.cur = 0; r.overflow = 0;
 }
 
@@ -82,13 +86,18 @@ OXD float uniform(Rng& r) {
         if (r.cur >= r.npre) { r.cur = 0; r.overflow++; }
         return r.pre[r.cur++];
     }
-    uint64_t block = r.pos >> 2;
-    if (block != r.blk) {
-        philox(uint32_t(block), uint32_t(block >> 32), r.p0, r.p1, r.k0, r.k1, r.b);
-        r.blk = block;
+    if (r.have == 0) {
+        uint64_t block = r.pos >> 2;
+        philox(uint32_t(block), uint32_t(block >> 32), r.p0, r.p1, r.k0, r.k1, r.b0, r.b1, r.b2, r.b3);
+        uint32_t skip = uint32_t(r.pos & 3);      // only the first block of a stream can start mid-block
+        if (skip >= 1u) { r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3; }
+        if (skip >= 2u) { r.b0 = r.b1; r.b1 = r.b2; }
+        if (skip >= 3u) { r.b0 = r.b1; }
+        r.have = 4 - int(skip);
     }
-    uint32_t j = uint32_t(r.pos & 3);
-    uint32_t x = j == 0 ? r.b[0] : (j == 1 ? r.b[1] : (j == 2 ? r.b[2] : r.b[3]));
+    uint32_t x = r.b0;
+    r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3;
+    r.have--;
     r.pos++;
     return float((x >> 8) + 1u) * (1.0f / 16777216.0f);
 }

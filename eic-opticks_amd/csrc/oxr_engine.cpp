@@ -147,6 +147,8 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     size_t smem = oxl::simulate_smem(P, block);
     if (smem > 64 * 1024) return fail(OXR_EINVAL, "LDS budget exceeded: lower block_size or lds_top_nodes");
     OXR_HIP(hipMemsetAsync(b.counters.p, 0, 8 * sizeof(unsigned long long), s));
+    // record slots past a photon's last point stay zero, as the reference's zeroed record array
+    if (P.record && n > 0) OXR_HIP(hipMemsetAsync(P.record, 0, size_t(n) * size_t(e.cfg.max_record) * 64, s));
     OXR_HIP(hipEventRecord(e.e0, s));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e.tune.blocks_per_cu) * std::max(1, e.num_cu);
@@ -176,6 +178,8 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
         st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         st->node_visits = ctr[2];
         st->prim_tests = ctr[3];
+        st->csg_node_bytes = ctr[5];
+        st->instance_visits = ctr[6];
     }
     e.num_hit = nhit;
     return OXR_OK;

@@ -8,6 +8,16 @@
 //   k_trace_rays, k_intersect_prim  geometry queries for tests and the simtrace-like API
 #include "oxr_kernels.h"
 
+// Minimum waves per SIMD the simulate kernels are compiled for (VGPR budget 512/N per
+// lane). The latency-bound traversal wants occupancy more than registers: measured on
+// MI355X, 3 for the persistent form and 4 for the plain form are fastest.
+#ifndef OXR_PERSIST_WAVES
+#define OXR_PERSIST_WAVES 3
+#endif
+#ifndef OXR_PLAIN_WAVES
+#define OXR_PLAIN_WAVES 4
+#endif
+
 namespace oxd {
 
 // ------------------------------------------------------------------ seeding
@@ -31,7 +41,9 @@ struct Lane {
     int bounce;
 };
 
+template <bool DBG>
 OXD void point(const Params& P, Lane& L) {
+    if (!DBG) return;
     if (P.record && L.bounce < P.max_record) store_photon(P.record + 4 * (L.idx * P.max_record + L.bounce), L.p);
     if (P.seq) {
         unsigned slot = unsigned(L.bounce);
@@ -45,6 +57,7 @@ OXD void point(const Params& P, Lane& L) {
 }
 
 // returns false for padding indices of a batched step
+template <bool DBG>
 OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
     L.idx = idx;
     const float4* gsb = P.gs;
@@ -70,7 +83,7 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
     }
     const float4* gs = gsb + 6 * gsi;
     rng_init(L.r, P.rng_seed, P.rng_offset, P.skip, (unsigned long long)evt, pidx);
-    if (P.pre) { L.r.pre = P.pre + (size_t)idx * P.npre; L.r.npre = P.npre; }
+    if (DBG && P.pre) { L.r.pre = P.pre + (size_t)idx * P.npre; L.r.npre = P.npre; }
     uint32_t gencode = __float_as_uint(gs[0].x);
     if (gencode == 6) gen_torch(L.p, gs, pidx, L.r);
     else if (gencode == 15 || gencode == 18) gen_cerenkov(L.p, gs, L.r, P.t);
@@ -80,38 +93,39 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
     set_index(L.p, pidx);
     L.sh0 = L.sh1 = L.sb0 = L.sb1 = 0ull;
     L.bounce = 0;
-    point(P, L);
+    point<DBG>(P, L);
     return true;
 }
 
 // one loop iteration of CSGOptiX7.cu:450-482; returns false when the photon is done
-template <bool FP64, bool COUNT>
+template <bool FP64, bool COUNT, bool DBG>
 OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode* top, int ntop,
-                     uint32_t& visits, uint32_t& tests, uint32_t& err) {
+                     uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return false;
     float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Hit h;
     bool found = trace<FP64, COUNT>(P.g, L.p.pos, L.p.mom, tmin, P.tmax, h, stk, stride, P.stack_cap, top, ntop,
-                                    visits, tests, err);
+                                    visits, tests, err, csgb, ivis);
     if (!found) return false;
     f3 normal = normalize(h.n);
     float lposcost = h.lpos.z / sqrtf(dot(h.lpos, h.lpos));
     int command = propagate(L.p, h, normal, lposcost, L.r, P.t, P.aligned);
     L.bounce++;
-    point(P, L);
+    point<DBG>(P, L);
     return command != BREAK;
 }
 
+template <bool DBG>
 OXD void end_photon(const Params& P, const Lane& L) {
-    if (P.seq) {
+    if (DBG && P.seq) {
         ulonglong2* s = P.seq + 2 * L.idx;
         s[0] = make_ulonglong2(L.sh0, L.sh1);
         s[1] = make_ulonglong2(L.sb0, L.sb1);
     }
     store_photon(P.photon + 4 * L.idx, L.p);
     if (P.hitflag) P.hitflag[L.idx] = (L.p.flagmask & P.hitmask) ? 1 : 0;
-    if (P.bounce_out) P.bounce_out[L.idx] = uint32_t(L.bounce);
-    if (P.overflow_out) P.overflow_out[L.idx] = L.r.overflow;
+    if (DBG && P.bounce_out) P.bounce_out[L.idx] = uint32_t(L.bounce);
+    if (DBG && P.overflow_out) P.overflow_out[L.idx] = L.r.overflow;
 }
 
 OXD void stage_top(const Params& P, BNode* top, int ntop) {
@@ -124,40 +138,44 @@ OXD unsigned long long wave_sum(unsigned long long v) {
 }
 
 OXD void flush_counters(const Params& P, unsigned long long steps, unsigned long long visits,
-                        unsigned long long tests, uint32_t err, bool count) {
+                        unsigned long long tests, uint32_t err, bool count,
+                        unsigned long long csgb = 0, unsigned long long ivis = 0) {
     steps = wave_sum(steps);
-    if (count) { visits = wave_sum(visits); tests = wave_sum(tests); }
+    if (count) { visits = wave_sum(visits); tests = wave_sum(tests); csgb = wave_sum(csgb); ivis = wave_sum(ivis); }
     uint64_t anyerr = __ballot(err != 0u);
     if ((threadIdx.x & 63) == 0) {
         atomicAdd(P.counters + 0, steps);
-        if (count) { atomicAdd(P.counters + 2, visits); atomicAdd(P.counters + 3, tests); }
+        if (count) {
+            atomicAdd(P.counters + 2, visits); atomicAdd(P.counters + 3, tests);
+            atomicAdd(P.counters + 5, csgb); atomicAdd(P.counters + 6, ivis);
+        }
         if (anyerr) atomicAdd(P.counters + 4, 1ull);
     }
 }
 
 // ------------------------------------------------------------------ plain form
-template <bool FP64, bool COUNT>
-__global__ void __launch_bounds__(256) k_simulate(Params P) {
+template <bool FP64, bool COUNT, bool DBG>
+__global__ void __launch_bounds__(256, OXR_PLAIN_WAVES) k_simulate(Params P) {
     extern __shared__ int4 smem4[];
     BNode* top = reinterpret_cast<BNode*>(smem4);
     int* stk = reinterpret_cast<int*>(top + P.ntop) + threadIdx.x;
     stage_top(P, top, P.ntop);
     __syncthreads();
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long steps = 0, visits = 0, tests = 0;
+    unsigned long long steps = 0, visits = 0, tests = 0, csgb = 0, ivis = 0;
     uint32_t err = 0;
     if (idx < P.num_photon) {
         Lane L;
-        if (begin_photon(P, L, idx)) {
-            uint32_t v = 0, t = 0;
-            while (step_photon<FP64, COUNT>(P, L, stk, blockDim.x, top, P.ntop, v, t, err)) {}
-            end_photon(P, L);
-            steps = (unsigned long long)L.bounce; visits = v; tests = t;
+        if (begin_photon<DBG>(P, L, idx)) {
+            uint32_t v = 0, t = 0, cb = 0, iv = 0;
+            while (step_photon<FP64, COUNT, DBG>(P, L, stk, blockDim.x, top, P.ntop, v, t, err, cb, iv)) {}
+            end_photon<DBG>(P, L);
+            steps = (unsigned long long)L.bounce; visits = v; tests = t; csgb = cb; ivis = iv;
         } else {
             P.hitflag[idx] = 0;
         }
     }
-    flush_counters(P, steps, visits, tests, err, COUNT);
+    flush_counters(P, steps, visits, tests, err, COUNT, csgb, ivis);
 }
 
 // ------------------------------------------------------------------ persistent form
@@ -165,8 +183,8 @@ __global__ void __launch_bounds__(256) k_simulate(Params P) {
 // refill_threshold lanes of a wave are idle (or all are), the wave claims that many
 // photon indices with one atomicAdd on counters[1] and the idle lanes generate them.
 // Every wave leaves the loop once the counter has passed num_photon and its lanes are idle.
-template <bool FP64>
-__global__ void __launch_bounds__(256) k_simulate_persist(Params P) {
+template <bool FP64, bool DBG>
+__global__ void __launch_bounds__(256, OXR_PERSIST_WAVES) k_simulate_persist(Params P) {
     extern __shared__ int4 smem4[];
     BNode* top = reinterpret_cast<BNode*>(smem4);
     int* stk = reinterpret_cast<int*>(top + P.ntop) + threadIdx.x;
@@ -177,7 +195,7 @@ __global__ void __launch_bounds__(256) k_simulate_persist(Params P) {
     bool active = false;
     bool drained = false;
     unsigned long long steps = 0;
-    uint32_t v = 0, t = 0, err = 0;
+    uint32_t v = 0, t = 0, err = 0, cb = 0, iv = 0;
     for (;;) {
         uint64_t idle = __ballot(!active);
         int nidle = __popcll(idle);
@@ -191,7 +209,7 @@ __global__ void __launch_bounds__(256) k_simulate_persist(Params P) {
                 unsigned rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
                 unsigned long long my = base + rank;
                 if (my < (unsigned long long)P.num_photon) {
-                    if (begin_photon(P, L, (long long)my)) active = true;
+                    if (begin_photon<DBG>(P, L, (long long)my)) active = true;
                     else P.hitflag[my] = 0;
                 }
             }
@@ -201,8 +219,8 @@ __global__ void __launch_bounds__(256) k_simulate_persist(Params P) {
             continue;
         }
         if (active) {
-            if (!step_photon<FP64, false>(P, L, stk, blockDim.x, top, P.ntop, v, t, err)) {
-                end_photon(P, L);
+            if (!step_photon<FP64, false, DBG>(P, L, stk, blockDim.x, top, P.ntop, v, t, err, cb, iv)) {
+                end_photon<DBG>(P, L);
                 steps += (unsigned long long)L.bounce;
                 active = false;
             }
@@ -293,9 +311,9 @@ __global__ void __launch_bounds__(256) k_trace_rays(Params P, const float4* __re
     if (i >= n) return;
     float4 a = rays[2 * i], b = rays[2 * i + 1];
     Hit h;
-    uint32_t v = 0, t = 0, err = 0;
+    uint32_t v = 0, t = 0, err = 0, cb = 0, iv = 0;
     bool found = trace<FP64, false>(P.g, mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), a.w, b.w, h, stk, blockDim.x,
-                                    P.stack_cap, top, P.ntop, v, t, err);
+                                    P.stack_cap, top, P.ntop, v, t, err, cb, iv);
     float lc = 0.f, lf = 0.f;
     if (found) {
         lc = h.lpos.z / sqrtf(dot(h.lpos, h.lpos));
@@ -334,16 +352,25 @@ size_t simulate_smem(const Params& P, int block) {
     return size_t(P.ntop) * sizeof(BNode) + size_t(block) * P.stack_cap * sizeof(int);
 }
 
+// debug outputs (seqhis, records, precooked streams) select the DBG kernel variants
+static bool wants_debug(const Params& P) {
+    return P.seq || P.record || P.pre || P.bounce_out || P.overflow_out;
+}
+
 hipError_t simulate_plain(hipStream_t s, const Params& P, int block, bool fp64, bool count) {
     if (P.num_photon <= 0) return hipSuccess;
     long long nb = (P.num_photon + block - 1) / block;
     size_t sm = simulate_smem(P, block);
-    if (fp64) {
-        if (count) hipLaunchKernelGGL((k_simulate<true, true>), dim3((unsigned)nb), dim3(block), sm, s, P);
-        else hipLaunchKernelGGL((k_simulate<true, false>), dim3((unsigned)nb), dim3(block), sm, s, P);
+    dim3 g((unsigned)nb), b(block);
+    if (count) {
+        if (fp64) hipLaunchKernelGGL((k_simulate<true, true, true>), g, b, sm, s, P);
+        else hipLaunchKernelGGL((k_simulate<false, true, true>), g, b, sm, s, P);
+    } else if (wants_debug(P)) {
+        if (fp64) hipLaunchKernelGGL((k_simulate<true, false, true>), g, b, sm, s, P);
+        else hipLaunchKernelGGL((k_simulate<false, false, true>), g, b, sm, s, P);
     } else {
-        if (count) hipLaunchKernelGGL((k_simulate<false, true>), dim3((unsigned)nb), dim3(block), sm, s, P);
-        else hipLaunchKernelGGL((k_simulate<false, false>), dim3((unsigned)nb), dim3(block), sm, s, P);
+        if (fp64) hipLaunchKernelGGL((k_simulate<true, false, false>), g, b, sm, s, P);
+        else hipLaunchKernelGGL((k_simulate<false, false, false>), g, b, sm, s, P);
     }
     return hipGetLastError();
 }
@@ -351,8 +378,14 @@ hipError_t simulate_plain(hipStream_t s, const Params& P, int block, bool fp64, 
 hipError_t simulate_persist(hipStream_t s, const Params& P, int block, int grid, bool fp64) {
     if (P.num_photon <= 0) return hipSuccess;
     size_t sm = simulate_smem(P, block);
-    if (fp64) hipLaunchKernelGGL((k_simulate_persist<true>), dim3(grid), dim3(block), sm, s, P);
-    else hipLaunchKernelGGL((k_simulate_persist<false>), dim3(grid), dim3(block), sm, s, P);
+    bool dbg = wants_debug(P);
+    if (fp64) {
+        if (dbg) hipLaunchKernelGGL((k_simulate_persist<true, true>), dim3(grid), dim3(block), sm, s, P);
+        else hipLaunchKernelGGL((k_simulate_persist<true, false>), dim3(grid), dim3(block), sm, s, P);
+    } else {
+        if (dbg) hipLaunchKernelGGL((k_simulate_persist<false, true>), dim3(grid), dim3(block), sm, s, P);
+        else hipLaunchKernelGGL((k_simulate_persist<false, false>), dim3(grid), dim3(block), sm, s, P);
+    }
     return hipGetLastError();
 }
 

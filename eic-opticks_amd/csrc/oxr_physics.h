@@ -4,6 +4,17 @@
 #pragma once
 #include "oxr_trace.h"
 
+// Generation runs once per photon; compiling it out of line keeps the bounce loop's code
+// smaller. OXR_GEN_INLINE=1 inlines it instead.
+#ifndef OXR_GEN_INLINE
+#define OXR_GEN_INLINE 1
+#endif
+#if OXR_GEN_INLINE
+#define OXG __device__ __forceinline__
+#else
+#define OXG __device__ __noinline__
+#endif
+
 namespace oxd {
 
 struct Ph {
@@ -283,7 +294,7 @@ OXD int propagate(Ph& p, const Hit& h, f3 normal, float lposcost, Rng& r, const 
 
 // ------------------------------------------------------------------ generation
 // storch::generate (sysrap/storch.h:189-516) plus the T_POINT_ISOTROPIC (8) extension
-OXD void gen_torch(Ph& p, const float4* gs, uint64_t photon_id, Rng& r) {
+OXG void gen_torch(Ph& p, const float4* gs, uint64_t photon_id, Rng& r) {
     float4 q0 = gs[0], q1 = gs[1], q2 = gs[2], q3 = gs[3], q4 = gs[4], q5 = gs[5];
     uint32_t numphoton = __float_as_uint(q0.w);
     uint32_t type = __float_as_uint(q5.w);
@@ -394,7 +405,7 @@ OXD void gen_torch(Ph& p, const float4* gs, uint64_t photon_id, Rng& r) {
 }
 
 // qcerenkov::generate (qudarap/qcerenkov.h:56-327)
-OXD void gen_cerenkov(Ph& p, const float4* gs, Rng& r, const Tables& t) {
+OXG void gen_cerenkov(Ph& p, const float4* gs, Rng& r, const Tables& t) {
     float4 q0 = gs[0], q1 = gs[1], q2 = gs[2], q3 = gs[3], q4 = gs[4], q5 = gs[5];
     f3 dp = mk(q2.x, q2.y, q2.z);
     f3 p0 = normalize(dp);
@@ -444,7 +455,7 @@ OXD void gen_cerenkov(Ph& p, const float4* gs, Rng& r, const Tables& t) {
 }
 
 // qscint::generate + mom_pol_wavelength (qudarap/qscint.h:55-145)
-OXD void gen_scint(Ph& p, const float4* gs, Rng& r, const Tables& t) {
+OXG void gen_scint(Ph& p, const float4* gs, Rng& r, const Tables& t) {
     float4 q1 = gs[1], q2 = gs[2], q3 = gs[3], q5 = gs[5];
     float u0 = uniform(r), u1 = uniform(r), u2 = uniform(r), u3 = uniform(r);
     float cost = 1.f - 2.f * u0;

@@ -63,7 +63,7 @@ OXD bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, f3 
 template <bool FP64, bool COUNT>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
-               uint32_t& visits, uint32_t& tests, uint32_t& err) {
+               uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
     float best = tmax;
     uint64_t best_key = ~0ull;
     bool found = false;
@@ -111,6 +111,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         int item = ~cur;
         if (at_top) {
             InstD in = g.inst[item];
+            if (COUNT) ivis++;
             inst = item;
             c0 = in.c0; c1 = in.c1; c2 = in.c2;
             ident = (in.info.z & 1) != 0;
@@ -132,7 +133,16 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         }
         // prim leaf: item is the local prim index within the instance's solid
         int2 pr = g.prim[primOffset + item];
-        if (COUNT) tests++;
+        if (COUNT) {
+            tests++;
+            unsigned tc0 = __float_as_uint(g.node[4 * pr.x + 3].z);
+            unsigned nn = tc0 >= CSG_LEAF ? 1u : __float_as_uint(g.node[4 * pr.x].x);
+            for (unsigned k = 0; k < nn; k++) {
+                float4 q3 = g.node[4 * (pr.x + int(k)) + 3];
+                csgb += 64u;
+                if (__float_as_uint(q3.z) >= CSG_LEAF && (__float_as_uint(q3.w) & 0x7fffffffu) != 0u) csgb += 64u;
+            }
+        }
         Is is;
         if (intersect_prim<FP64>(is, g.node, pr.x, g.plan, g.itra, tmin, o, d)) {
             float t = is.w;

@@ -22,7 +22,7 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, 'liboxr.so')
+LIB_PATH = os.environ.get('OXR_LIB') or os.path.join(HERE, 'liboxr.so')
 
 _lib = None
 

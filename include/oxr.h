@@ -141,6 +141,9 @@ typedef struct oxr_launch_stats {
     double   total_ms;        /* launch call wall time                                  */
     uint64_t node_visits;     /* BVH node visits (when counting is enabled)             */
     uint64_t prim_tests;      /* CSG prim intersect calls (when counting is enabled)    */
+    uint64_t csg_node_bytes;  /* 64 B per CSG node of each tested prim + 64 B per
+                                 transformed leaf (when counting is enabled)            */
+    uint64_t instance_visits; /* instance records read on entering a BLAS (counting)    */
 } oxr_launch_stats;
 
 typedef struct oxr_engine_info {

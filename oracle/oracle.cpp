@@ -139,6 +139,11 @@ struct Ctx {
     const float* inst; int64_t num_inst;
     std::vector<Q4x3<double>> w2o;    // per-instance world-to-object (double)
     std::vector<uint8_t> inst_identity_transform;
+    // optional conservative culling (world box per instance, object box per prim); off by
+    // default, validated against the unculled loop in tests
+    int cull = 0;
+    std::vector<double> inst_box;     // 6 per instance
+    std::vector<double> prim_box;     // 6 per prim
     // physics
     const float* bnd; int32_t num_bnd, num_wl; float wl_min, wl_step;
     const uint32_t* optical;
@@ -743,6 +748,22 @@ template <> inline float mat<float>(const Q4x3<double>& m, int r, int k) { retur
 // of CSGOptiX7.cu:110-216 (closest hit in (tmin,tmax], anyhit disabled) with the IS/CH/MS
 // programs of CSGOptiX7.cu:655-940. Ties at equal t resolve to the smaller
 // (instance index, prim index), the engine's deterministic rule.
+// conservative slab test in double against a box expanded by 1e-4 relative + 1e-2 mm
+inline bool box_may_hit(const double* b, double ox, double oy, double oz, double dx, double dy, double dz, double tmax) {
+    double t0 = 0.0, t1 = tmax;
+    const double o[3] = {ox, oy, oz}, d[3] = {dx, dy, dz};
+    for (int k = 0; k < 3; k++) {
+        double m = std::max(std::fabs(b[k]), std::fabs(b[3 + k]));
+        double lo = b[k] - 1e-4 * m - 1e-2, hi = b[3 + k] + 1e-4 * m + 1e-2;
+        if (d[k] == 0.0) { if (o[k] < lo || o[k] > hi) return false; continue; }
+        double a = (lo - o[k]) / d[k], bb = (hi - o[k]) / d[k];
+        if (a > bb) std::swap(a, bb);
+        t0 = std::max(t0, a); t1 = std::min(t1, bb);
+        if (t0 > t1) return false;
+    }
+    return true;
+}
+
 template <typename T>
 bool trace(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
     bool found = false;
@@ -758,6 +779,8 @@ bool trace(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
         const int32_t* so = c.solid + 12 * size_t(gas);
         int numPrim = so[4], primOffset = so[5];
         bool ident = c.inst_identity_transform[i] != 0;
+        if (c.cull && !box_may_hit(&c.inst_box[6 * size_t(i)], double(pos.x), double(pos.y), double(pos.z),
+                                   double(mom.x), double(mom.y), double(mom.z), double(best))) continue;
         V3<T> o = pos, d = mom;
         const Q4x3<double>& w = c.w2o[i];
         if (!ident) {
@@ -773,6 +796,8 @@ bool trace(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
             int32_t nodeOffset; std::memcpy(&nodeOffset, pr + 1, 4);
             uint32_t gpi = f2u(pr[15]);
             const Node* root = c.node + nodeOffset;
+            if (c.cull && !box_may_hit(&c.prim_box[6 * size_t(primOffset + p)], double(o.x), double(o.y), double(o.z),
+                                       double(d.x), double(d.y), double(d.z), double(best))) continue;
             Isect<T> is;
             if (!intersect_prim(c, is, root, tmin, o, d)) continue;
             T t = is.w;
@@ -1391,6 +1416,36 @@ void* orc_create(const oxr_geometry* g, const oxr_physics* p, const oxr_config* 
 void orc_destroy(void* c) { delete static_cast<orc::Ctx*>(c); }
 
 void orc_set_config(void* c, const oxr_config* cfg) { static_cast<orc::Ctx*>(c)->cfg = *cfg; }
+
+// enable conservative box culling in trace (instance world boxes, prim object boxes)
+void orc_set_cull(void* cv, int32_t on) {
+    orc::Ctx& c = *static_cast<orc::Ctx*>(cv);
+    c.cull = on;
+    if (!on || !c.inst_box.empty()) return;
+    c.prim_box.resize(6 * size_t(c.num_prim));
+    for (int64_t p = 0; p < c.num_prim; p++) {
+        const float* q = c.prim + 16 * p;
+        for (int k = 0; k < 6; k++) c.prim_box[6 * p + k] = q[8 + k];
+    }
+    c.inst_box.resize(6 * size_t(c.num_inst));
+    for (int64_t i = 0; i < c.num_inst; i++) {
+        const float* q = c.inst + 16 * i;
+        int32_t gas; std::memcpy(&gas, q + 7, 4);
+        const int32_t* so = c.solid + 12 * size_t(gas);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int p = 0; p < so[4]; p++) {
+            const double* b = &c.prim_box[6 * size_t(so[5] + p)];
+            for (int cc = 0; cc < 8; cc++) {
+                double pt[3] = {(cc & 1) ? b[3] : b[0], (cc & 2) ? b[4] : b[1], (cc & 4) ? b[5] : b[2]};
+                for (int k = 0; k < 3; k++) {
+                    double w = pt[0] * q[k] + pt[1] * q[4 + k] + pt[2] * q[8 + k] + q[12 + k];
+                    lo[k] = std::min(lo[k], w); hi[k] = std::max(hi[k], w);
+                }
+            }
+        }
+        for (int k = 0; k < 3; k++) { c.inst_box[6 * i + k] = lo[k]; c.inst_box[6 * i + 3 + k] = hi[k]; }
+    }
+}
 
 // seed expansion: QEvt_count_genstep_photons_and_fill_seed_buffer (qudarap/QEvt.cu:181-237)
 int64_t orc_fill_seed(const float* gs, int64_t num_gs, int32_t* seed, int64_t cap) {

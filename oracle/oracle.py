@@ -37,6 +37,7 @@ def lib():
         L.orc_create.argtypes = [C.POINTER(abi.Geometry), C.POINTER(abi.Physics), C.POINTER(abi.Config)]
         L.orc_destroy.argtypes = [vp]
         L.orc_set_config.argtypes = [vp, C.POINTER(abi.Config)]
+        L.orc_set_cull.argtypes = [vp, i32]
         L.orc_fill_seed.restype = i64
         L.orc_fill_seed.argtypes = [vp, i64, vp, i64]
         L.orc_intersect_prim.argtypes = [vp, i32, vp, i64, vp, vp, i32]
@@ -114,6 +115,9 @@ class Oracle:
                 lib().orc_destroy(self.h)
         except Exception:
             pass
+
+    def set_cull(self, on=True):
+        lib().orc_set_cull(self.h, int(on))
 
     def set_config(self, cfg):
         self.cfg = cfg

@@ -1,0 +1,57 @@
+"""CPU checks of the C ABI: the library loads and exports every declared symbol, and the
+host-only entry points behave (no GPU calls)."""
+import ctypes as C
+import os
+import re
+
+from oxr import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, 'include', 'oxr.h')).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(oxr_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_every_declared_symbol_is_exported(engine_mod):
+    L = engine_mod.lib()
+    names = declared_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header(engine_mod):
+    # sizes implied by include/oxr.h on LP64
+    assert C.sizeof(abi.Geometry) == 96
+    assert C.sizeof(abi.Config) == 80
+    assert C.sizeof(abi.LaunchStats) == 80
+    assert C.sizeof(abi.Tuning) == 16
+    assert C.sizeof(abi.RequestResult) == 64
+
+
+def test_config_defaults_are_reference_defaults(engine_mod):
+    """sysrap/SEventConfig.cc:45-106: MaxBounce 31, PropagateEpsilon(0) 0.05,
+    Epsilon0Mask TO,CK,SI,SC,RE, EventSkipahead 100000, HitMask SD, MaxTime 1e27."""
+    c = abi.Config()
+    engine_mod.lib().oxr_config_defaults(C.byref(c))
+    F = abi.FLAGS
+    assert c.max_bounce == 31
+    assert abs(c.tmin - 0.05) < 1e-7 and abs(c.tmin0 - 0.05) < 1e-7
+    assert c.eps0_mask == F['TORCH'] | F['CERENKOV'] | F['SCINTILLATION'] | F['BULK_SCATTER'] | F['BULK_REEMIT']
+    assert c.skipahead_event == 100000
+    assert c.hitmask == F['SURFACE_DETECT']
+    assert c.max_time > 1e26
+    py = abi.default_config()
+    for k, _ in abi.Config._fields_:
+        assert getattr(py, k) == getattr(c, k), k
+
+
+def test_version_and_error_string(engine_mod):
+    L = engine_mod.lib()
+    assert L.oxr_version().decode().startswith('oxr')
+    # a null engine is rejected on the host with a message, before any device call
+    rc = L.oxr_engine_set_config(None, None)
+    assert rc == -1 and b'null' in L.oxr_last_error()

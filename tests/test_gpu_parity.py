@@ -1,0 +1,261 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle on the same
+seeded inputs. Integer/flag/index outputs and the FP32 photon arrays must agree bit for bit."""
+import numpy as np
+import pytest
+
+from oxr import abi, gensteps as G, scenes
+from oxr.csg import Foundry, box3, sphere, zsphere, cylinder, cone, convexpolyhedron, phicut, scale, translate, rotate_z_to
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    'nested_boxes': (lambda: scenes.nested_boxes(), 0.2),
+    'csi_calorimeter': (lambda: scenes.csi_calorimeter(), 0.01),
+    'instanced_csg': (lambda: scenes.instanced_csg(num_inst=2000), 0.002),
+    'thin_shells': (lambda: scenes.thin_shells(num_inst=300), 0.002),
+}
+_cache = {}
+
+
+def scene(name):
+    if name not in _cache:
+        _cache[name] = SCENES[name][0]()
+    return _cache[name]
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def run_engine(engine_mod, s, gs, cfg, form=abi.OXR_FORM_PLAIN, inp=None, pre=None, event_index=0, slot=0):
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        if inp is not None:
+            e.set_input_photons(inp)
+        if pre is not None:
+            e.set_precooked(pre)
+        n = e.set_gensteps(gs, event_index=event_index, slot_offset=slot)
+        st = e.launch(form)
+        out = dict(photon=e.photons(), hit=e.hits(), seed=e.seed(), stats=st, n=n)
+        if cfg.seq_enabled:
+            out['seq'] = e.seq()
+        if cfg.max_record:
+            out['record'] = e.get(abi.OXR_GET_RECORD)
+    return out
+
+
+def run_oracle(oracle_lib, s, gs, cfg, inp=None, pre=None, event_index=0, slot=0, cull=False):
+    o = oracle_lib.Oracle(s.geometry, s.physics, cfg)
+    if cull:
+        o.set_cull(True)
+    return o.simulate(gs, event_index=event_index, slot_offset=slot, precooked=pre, input_photons=inp,
+                      max_record=cfg.max_record)
+
+
+@pytest.mark.parametrize('name', list(SCENES))
+@pytest.mark.parametrize('schedule', [abi.OXR_RNG_PRODUCTION, abi.OXR_RNG_ALIGNED])
+def test_simulate_bitexact(engine_mod, oracle_lib, name, schedule):
+    s = scene(name)
+    cfg = abi.default_config(**s.config, seq_enabled=1, rng_schedule=schedule)
+    gs = s.gensteps(scale_photons=SCENES[name][1])
+    g = run_engine(engine_mod, s, gs, cfg)
+    r = run_oracle(oracle_lib, s, gs, cfg, cull=True)
+    print(name, schedule, g['n'], 'photons', g['stats']['bounce_steps'], 'steps', g['stats']['num_hit'], 'hits')
+    assert g['n'] == r['total'] > 0
+    assert np.array_equal(bits(g['photon']), bits(r['photon']))
+    assert np.array_equal(g['seq'], r['seq'])
+    assert g['stats']['bounce_steps'] == int(r['bounce'].sum())
+    # hits are the SD photons in photon order (SU::copy_if semantics)
+    sel = (bits(r['photon'])[:, 3, 3] & cfg.hitmask) != 0
+    assert np.array_equal(bits(g['hit']), bits(r['photon'][sel]))
+
+
+@pytest.mark.parametrize('name', list(SCENES))
+def test_persistent_equals_plain(engine_mod, name):
+    s = scene(name)
+    cfg = abi.default_config(**s.config, seq_enabled=1)
+    gs = s.gensteps(scale_photons=SCENES[name][1])
+    a = run_engine(engine_mod, s, gs, cfg, abi.OXR_FORM_PLAIN)
+    b = run_engine(engine_mod, s, gs, cfg, abi.OXR_FORM_PERSISTENT)
+    assert np.array_equal(bits(a['photon']), bits(b['photon']))
+    assert np.array_equal(a['seq'], b['seq'])
+    assert np.array_equal(bits(a['hit']), bits(b['hit']))
+    assert a['stats']['bounce_steps'] == b['stats']['bounce_steps']
+
+
+def test_raindrop_known_answer(engine_mod, oracle_lib):
+    """tests/test_GPUPhotonFileSource.sh: 10 input photons -> 10 hits, wavelengths preserved."""
+    s = scenes.raindrop()
+    cfg = abi.default_config(**s.config, seq_enabled=1)
+    inp = scenes.raindrop_input_photons()
+    g = run_engine(engine_mod, s, s.gensteps(), cfg, inp=inp)
+    assert len(g['hit']) == 10
+    assert sorted(g['hit'][:, 2, 3].tolist()) == [420] * 3 + [450] * 2 + [500] * 5
+    r = run_oracle(oracle_lib, s, s.gensteps(), cfg, inp=inp)
+    assert np.array_equal(bits(g['photon']), bits(r['photon']))
+
+
+def test_precooked_streams(engine_mod, oracle_lib):
+    s = scene('nested_boxes')
+    cfg = abi.default_config(**s.config, seq_enabled=1)
+    gs = s.gensteps(scale_photons=0.02)
+    n = G.num_photons(gs)
+    rng = np.random.default_rng(7)
+    for pre in (rng.uniform(1e-7, 1.0, size=(n, 256)).astype(np.float32),     # (N,16,16) precooked
+                np.full((n, 16), 0.5, dtype=np.float32),                        # degenerate, wraps
+                np.tile(np.float32([0.3, 0.7, 0.9]), (n, 1)),                   # short cycle
+                np.full((n, 8), 1.0, dtype=np.float32)):                        # u=1: zero distances
+        g = run_engine(engine_mod, s, gs, cfg, pre=pre)
+        r = run_oracle(oracle_lib, s, gs, cfg, pre=pre)
+        assert np.array_equal(bits(g['photon']), bits(r['photon']))
+        assert np.array_equal(g['seq'], r['seq'])
+
+
+def test_seed_expansion(engine_mod, oracle_lib):
+    s = scene('nested_boxes')
+    counts = np.random.default_rng(8).integers(0, 40, size=500)
+    counts[[0, 7, 499]] = 0
+    gs = G.torch(counts)
+    g = run_engine(engine_mod, s, gs, abi.default_config(**s.config))
+    assert np.array_equal(g['seed'], oracle_lib.fill_seed(gs))
+    assert np.array_equal(g['seed'], np.repeat(np.arange(500), counts))
+
+
+def test_event_index_and_slot_offset(engine_mod, oracle_lib):
+    s = scene('csi_calorimeter')
+    cfg = abi.default_config(**s.config, seq_enabled=1)
+    gs = s.gensteps(scale_photons=0.003)
+    g = run_engine(engine_mod, s, gs, cfg, event_index=3, slot=1000)
+    r = run_oracle(oracle_lib, s, gs, cfg, event_index=3, slot=1000)
+    assert np.array_equal(bits(g['photon']), bits(r['photon']))
+    assert np.all(bits(g['photon'])[:, 3, 2] == 1000 + np.arange(g['n']))
+    g0 = run_engine(engine_mod, s, gs, cfg, event_index=0, slot=1000)
+    assert not np.array_equal(bits(g['photon']), bits(g0['photon']))
+
+
+def test_slicing_matches_single_launch(engine_mod):
+    """QSim::simulate genstep slicing (QSim.cc:455-526): multi-launch == single launch."""
+    s = scene('csi_calorimeter')
+    gs = s.gensteps(scale_photons=0.005)
+    cfg = abi.default_config(**s.config)
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        h1, st1 = e.simulate(gs)
+        e.set_config(max_photon=1500)
+        h2, st2 = e.simulate(gs)
+    assert st1['num_photon'] == st2['num_photon']
+    assert np.array_equal(bits(h1), bits(h2))
+
+
+def test_records_match(engine_mod, oracle_lib):
+    s = scene('nested_boxes')
+    cfg = abi.default_config(**s.config, seq_enabled=1, max_record=10)
+    gs = s.gensteps(scale_photons=0.01)
+    g = run_engine(engine_mod, s, gs, cfg)
+    r = run_oracle(oracle_lib, s, gs, cfg)
+    assert np.array_equal(bits(g['record']), bits(r['record']))
+
+
+def test_fp64_roots_mode(engine_mod, oracle_lib):
+    s = scene('thin_shells')
+    cfg = abi.default_config(**s.config, seq_enabled=1, fp64_roots=1)
+    gs = s.gensteps(scale_photons=0.002)
+    g = run_engine(engine_mod, s, gs, cfg)
+    r = run_oracle(oracle_lib, s, gs, cfg, cull=True)
+    assert np.array_equal(bits(g['photon']), bits(r['photon']))
+    assert np.array_equal(g['seq'], r['seq'])
+
+
+def test_empty_event_and_errors(engine_mod):
+    s = scene('nested_boxes')
+    cfg = abi.default_config(**s.config)
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        assert e.set_gensteps(np.zeros((0, 6, 4), np.float32)) == 0
+        st = e.launch()
+        assert st['num_photon'] == 0 and len(e.hits()) == 0
+        bad = G.cerenkov([10], [[0, 0, 0]], [[0, 0, 1.0]], 1.0, 0.9, 10_000, 200.0, 700.0, 1.5, 10.0)
+        with pytest.raises(engine_mod.OxrError) as ex:
+            e.set_gensteps(bad)
+        assert ex.value.code == -1
+        e.set_gensteps(G.input_photon(5))
+        with pytest.raises(engine_mod.OxrError) as ex:
+            e.launch()
+        assert ex.value.code == -5
+        # the engine keeps working after rejected calls
+        e.set_gensteps(s.gensteps(scale_photons=0.001))
+        assert e.launch()['num_photon'] == 100
+
+
+# ---------------------------------------------------------------- geometry queries
+def zoo():
+    f = Foundry()
+    f.add_material('M', 1.0)
+    b = f.boundary('M', None, None, 'M')
+    f.begin_solid('zoo')
+    tet = [(1, 1, 1, 60), (-1, -1, 1, 60), (-1, 1, -1, 60), (1, -1, -1, 60)]
+    tet = [tuple(list(np.array(p[:3]) / np.linalg.norm(p[:3])) + [p[3]]) for p in tet]
+    prims = [
+        sphere(100.0),
+        sphere(50.0, 10.0, -20.0, 5.0),
+        zsphere(100.0, -40.0, 70.0),
+        cylinder(40.0, -80.0, 90.0),
+        box3(100.0, 60.0, 30.0),
+        box3(80.0).moved(rotate_z_to([1.0, 2.0, 3.0]) @ translate(5.0, -7.0, 9.0)),
+        cone(80.0, -60.0, 20.0, 70.0),
+        convexpolyhedron(tet, (-110, -110, -110, 110, 110, 110)),
+        phicut(0.25, 1.75) & sphere(90.0),
+        scenes.pmt_like_solid(),
+        (sphere(80.0, 30.0, 0.0, 0.0) & box3(120.0)) - cylinder(20.0, -200.0, 200.0),
+        (box3(100.0) | sphere(70.0, 0.0, 0.0, 60.0)) - (cylinder(30.0, -100.0, 100.0) | box3(20.0, 200.0, 20.0)),
+    ]
+    for t in prims:
+        f.add_prim(t, b)
+    f.end_solid()
+    f.add_instance(np.eye(4), 0)
+    return f, len(prims)
+
+
+def random_rays(rng, n, ext):
+    o = rng.normal(size=(n, 3))
+    o = o / np.linalg.norm(o, axis=1)[:, None] * rng.uniform(0.0, 3.0 * ext, size=(n, 1))
+    tgt = rng.uniform(-ext, ext, size=(n, 3))
+    d = tgt - o
+    k = n // 8
+    d[:k] = np.eye(3)[rng.integers(0, 3, size=k)] * rng.choice([-1, 1], size=(k, 1))   # axis-aligned
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    r = np.zeros((n, 2, 4), dtype=np.float32)
+    r[:, 0, :3] = o
+    r[:, 0, 3] = rng.choice([0.0, 0.05, 1.0], size=n)
+    r[:, 1, :3] = d
+    r[:, 1, 3] = 1e6
+    return r
+
+
+@pytest.mark.parametrize('fp64', [0, 1])
+def test_intersect_prim_bitexact(engine_mod, oracle_lib, fp64):
+    f, nprim = zoo()
+    cfg = abi.default_config(fp64_roots=fp64)
+    rng = np.random.default_rng(9)
+    o = oracle_lib.Oracle(f.arrays(), f.physics(), cfg)
+    with engine_mod.Engine(f.arrays(), f.physics(), cfg) as e:
+        for p in range(nprim):
+            rays = random_rays(rng, 4000, 150.0)
+            gi, gv = e.intersect_prim(p, rays)
+            oi, ov = o.intersect_prim(p, rays, 0)
+            assert np.array_equal(gv, ov), p
+            assert np.array_equal(bits(gi), bits(oi)), p
+            assert gv.mean() > 0.05, (p, gv.mean())
+
+
+@pytest.mark.parametrize('name', ['nested_boxes', 'csi_calorimeter', 'instanced_csg', 'thin_shells'])
+def test_trace_bitexact(engine_mod, oracle_lib, name):
+    s = scene(name)
+    cfg = abi.default_config(**s.config)
+    rng = np.random.default_rng(10)
+    ext = {'nested_boxes': 120.0, 'csi_calorimeter': 200.0, 'instanced_csg': 19500.0, 'thin_shells': 20000.0}[name]
+    rays = random_rays(rng, 20000, ext)
+    o = oracle_lib.Oracle(s.geometry, s.physics, cfg)
+    o.set_cull(True)
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        g = e.trace(rays)
+    r = o.trace(rays)
+    assert np.array_equal(bits(g), bits(r))
+    assert (bits(g)[:, 1, 3] & 0xffff != 0xffff).mean() > 0.5
