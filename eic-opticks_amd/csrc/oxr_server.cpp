@@ -1,11 +1,18 @@
 // oxr_server.cpp -- serving form: a request loop that keeps geometry, tables and the RNG
 // definition resident and batches genstep requests from many events into shared device
-// steps (continuous batching). Requests are admitted as the HBM budget and the active
-// slot count allow; each step gives every active request a share of the step's photon
-// budget, shortest remaining first; one launch covers all of them through a segment
-// table; each request completes as soon as its last photon has run and its hits are
-// returned on their own. Photon random streams are keyed by (event_index, photon index
-// within the event), so a request's result does not depend on what it was batched with.
+// steps (continuous batching).
+//
+//  * Admission: requests enter in arrival order while the active-slot count and the HBM
+//    budget (64 B/photon state + 4 B seed + reserved hit bytes + gensteps) allow.
+//  * Steps: every step gives the active requests a share of the step's photon budget,
+//    shortest remaining first; one persistent (or plain) launch covers all of them via
+//    a segment table; segments start on compaction-chunk boundaries so each request's
+//    hits are a contiguous range of the ordered hit array.
+//  * Overlap: two step slots on two streams. While the device runs step k, the host
+//    copies out and distributes the hits of step k-1; a request completes as soon as
+//    the last step holding its photons has been distributed.
+// Photon random streams are keyed by (event_index, photon index within the event), so a
+// request's result does not depend on what it was batched with.
 #include "oxr_engine.h"
 #include <algorithm>
 #include <chrono>
@@ -32,16 +39,34 @@ struct Req {
     DBuf<float4> d_gs;
     DBuf<int> d_seed;
     DBuf<long long> d_off;
-    int64_t dispatched = 0;
+    int64_t dispatched = 0;   // photons placed in launched steps
+    int inflight = 0;         // launched steps holding its photons, not yet distributed
     std::vector<float> hits;
     int64_t nhit = 0;
     void free_device() { d_gs.release(); d_seed.release(); d_off.release(); }
 };
 
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventBuffers buf;
+    DBuf<oxd::Seg> d_segs;
+    std::vector<oxd::Seg> segs;
+    std::vector<Req*> owner;
+    std::vector<int64_t> take;
+    long long* h_offs = nullptr;    // pinned
+    size_t h_offs_n = 0;
+    float* h_hits = nullptr;        // pinned staging
+    size_t h_hits_n = 0;            // floats
+    int64_t N = 0;
+    bool busy = false;
+    uint64_t seqno = 0;
+    std::chrono::steady_clock::time_point t0;
+};
+
 struct Server {
     Engine* e = nullptr;
     oxr_server_config cfg{};
-    hipStream_t stream = nullptr;
     std::thread th;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -51,16 +76,21 @@ struct Server {
     std::map<int64_t, std::unique_ptr<Req>> finished;
     int64_t budget = 0, used = 0;
     bool stop = false;
-    EventBuffers step;
-    DBuf<oxd::Seg> d_segs;
-    hipEvent_t e_ev0 = nullptr, e_ev1 = nullptr;
+    Slot slot[2];
+    uint64_t next_seq = 0;
+    hipStream_t admit_stream = nullptr;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double stat_steps = 0, stat_photons = 0, stat_busy_ms = 0, stat_kernel_ms = 0, stat_bounce = 0, stat_hits = 0;
     double clock() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
     void loop();
     int admit_upload(Req& r);
-    int run_step();
+    int launch_step(Slot& s);
+    int finalize_step(Slot& s);
     void complete(std::unique_ptr<Req>& r, int status);
+    bool dispatchable() const {
+        for (auto& r : active) if (r->dispatched < r->num_photon) return true;
+        return false;
+    }
 };
 
 int Server::admit_upload(Req& r) {
@@ -69,10 +99,11 @@ int Server::admit_upload(Req& r) {
     OXR_HIP(r.d_off.reserve(size_t(r.num_gs) + 1));
     OXR_HIP(r.d_seed.reserve(np));
     if (r.num_gs > 0) {
-        OXR_HIP(hipMemcpyAsync(r.d_gs.p, r.gs.data(), size_t(r.num_gs) * 96, hipMemcpyHostToDevice, stream));
-        OXR_HIP(hipMemcpyAsync(r.d_off.p, r.off.data(), (size_t(r.num_gs) + 1) * sizeof(long long), hipMemcpyHostToDevice, stream));
-        OXR_HIP(oxl::fill_seed(stream, r.d_off.p, int(r.num_gs), r.d_seed.p, r.num_photon));
+        OXR_HIP(hipMemcpyAsync(r.d_gs.p, r.gs.data(), size_t(r.num_gs) * 96, hipMemcpyHostToDevice, admit_stream));
+        OXR_HIP(hipMemcpyAsync(r.d_off.p, r.off.data(), (size_t(r.num_gs) + 1) * sizeof(long long), hipMemcpyHostToDevice, admit_stream));
+        OXR_HIP(oxl::fill_seed(admit_stream, r.d_off.p, int(r.num_gs), r.d_seed.p, r.num_photon));
     }
+    OXR_HIP(hipStreamSynchronize(admit_stream));
     return OXR_OK;
 }
 
@@ -87,77 +118,103 @@ void Server::complete(std::unique_ptr<Req>& r, int status) {
     cv_done.notify_all();
 }
 
-int Server::run_step() {
+int Server::launch_step(Slot& s) {
     const long long CH = 4096;
     std::vector<Req*> order;
     for (auto& r : active) if (r->dispatched < r->num_photon) order.push_back(r.get());
     std::sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
         return (a->num_photon - a->dispatched) < (b->num_photon - b->dispatched);
     });
-    std::vector<oxd::Seg> segs;
-    std::vector<Req*> owner;
-    std::vector<int64_t> take;
+    s.segs.clear(); s.owner.clear(); s.take.clear();
     long long cursor = 0;
     int64_t left = std::max<int64_t>(cfg.max_step_photons, CH);
     for (Req* r : order) {
         if (left <= 0) break;
         int64_t t = std::min(left, r->num_photon - r->dispatched);
-        oxd::Seg s{};
-        s.start = cursor; s.count = t;
-        s.photon0 = uint64_t(r->dispatched);
-        s.gs = r->d_gs.p; s.seed = r->d_seed.p;
-        s.event_index = r->event_index;
-        segs.push_back(s); owner.push_back(r); take.push_back(t);
+        oxd::Seg g{};
+        g.start = cursor; g.count = t;
+        g.photon0 = uint64_t(r->dispatched);
+        g.gs = r->d_gs.p; g.seed = r->d_seed.p;
+        g.event_index = r->event_index;
+        s.segs.push_back(g); s.owner.push_back(r); s.take.push_back(t);
         cursor += ((t + CH - 1) / CH) * CH;
         left -= t;
     }
-    if (segs.empty()) return OXR_OK;
-    auto w0 = std::chrono::steady_clock::now();
-    int64_t N = cursor;
-    int rc = ensure_event_capacity(*e, step, N);
+    if (s.segs.empty()) return OXR_OK;
+    s.t0 = std::chrono::steady_clock::now();
+    s.N = cursor;
+    int rc = ensure_event_capacity(*e, s.buf, s.N);
     if (rc) return rc;
-    OXR_HIP(d_segs.reserve(segs.size()));
-    OXR_HIP(hipMemcpyAsync(d_segs.p, segs.data(), segs.size() * sizeof(oxd::Seg), hipMemcpyHostToDevice, stream));
+    int nb = oxl::compact_blocks(s.N);
+    if (s.h_offs_n < size_t(nb) + 1) {
+        if (s.h_offs) (void)hipHostFree(s.h_offs);
+        s.h_offs = nullptr; s.h_offs_n = 0;
+        OXR_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_offs), (size_t(nb) + 1) * sizeof(long long), hipHostMallocDefault));
+        s.h_offs_n = size_t(nb) + 1;
+    }
+    OXR_HIP(s.d_segs.reserve(s.segs.size()));
+    OXR_HIP(hipMemcpyAsync(s.d_segs.p, s.segs.data(), s.segs.size() * sizeof(oxd::Seg), hipMemcpyHostToDevice, s.stream));
     int form = cfg.form;
     int block = form == OXR_FORM_PERSISTENT ? e->tune.block_size : 256;
-    oxd::Params P = e->params(step, N, 0, 0, e->ntop_for(block));
+    oxd::Params P = e->params(s.buf, s.N, 0, 0, e->ntop_for(block));
     P.seq = nullptr; P.record = nullptr;
-    P.segs = d_segs.p; P.nseg = int(segs.size());
-    OXR_HIP(hipMemsetAsync(step.counters.p, 0, 8 * sizeof(unsigned long long), stream));
-    OXR_HIP(hipEventRecord(e_ev0, stream));
+    P.segs = s.d_segs.p; P.nseg = int(s.segs.size());
+    OXR_HIP(hipMemsetAsync(s.buf.counters.p, 0, 8 * sizeof(unsigned long long), s.stream));
+    OXR_HIP(hipEventRecord(s.e0, s.stream));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e->tune.blocks_per_cu) * std::max(1, e->num_cu);
-        OXR_HIP(oxl::simulate_persist(stream, P, block, grid, e->cfg.fp64_roots != 0));
+        OXR_HIP(oxl::simulate_persist(s.stream, P, block, grid, e->cfg.fp64_roots != 0));
     } else {
-        OXR_HIP(oxl::simulate_plain(stream, P, block, e->cfg.fp64_roots != 0, false));
+        OXR_HIP(oxl::simulate_plain(s.stream, P, block, e->cfg.fp64_roots != 0, false));
     }
-    OXR_HIP(hipEventRecord(e_ev1, stream));
-    OXR_HIP(oxl::compact(stream, step.hitflag.p, step.photon.p, N, step.counts.p, step.offs.p, step.hit.p, N));
-    int nb = oxl::compact_blocks(N);
-    std::vector<long long> offs(size_t(nb) + 1);
-    unsigned long long ctr[8];
-    OXR_HIP(hipMemcpyAsync(offs.data(), step.offs.p, offs.size() * sizeof(long long), hipMemcpyDeviceToHost, stream));
-    OXR_HIP(hipMemcpyAsync(ctr, step.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, stream));
-    OXR_HIP(hipStreamSynchronize(stream));
+    OXR_HIP(hipEventRecord(s.e1, s.stream));
+    OXR_HIP(oxl::compact(s.stream, s.buf.hitflag.p, s.buf.photon.p, s.N, s.buf.counts.p, s.buf.offs.p, s.buf.hit.p, s.N));
+    OXR_HIP(hipMemcpyAsync(s.h_offs, s.buf.offs.p, (size_t(nb) + 1) * sizeof(long long), hipMemcpyDeviceToHost, s.stream));
+    for (size_t k = 0; k < s.segs.size(); k++) {
+        s.owner[k]->dispatched += s.take[k];
+        s.owner[k]->inflight += 1;
+    }
+    s.busy = true;
+    s.seqno = next_seq++;
+    return OXR_OK;
+}
+
+int Server::finalize_step(Slot& s) {
+    const long long CH = 4096;
+    s.busy = false;
+    for (auto* r : s.owner) r->inflight -= 1;
+    OXR_HIP(hipStreamSynchronize(s.stream));
+    int nb = oxl::compact_blocks(s.N);
+    long long nh = s.h_offs[nb];
+    if (nh > 0) {
+        size_t need = size_t(nh) * 16;
+        if (s.h_hits_n < need) {
+            if (s.h_hits) (void)hipHostFree(s.h_hits);
+            s.h_hits = nullptr; s.h_hits_n = 0;
+            OXR_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hits), need * sizeof(float), hipHostMallocDefault));
+            s.h_hits_n = need;
+        }
+        OXR_HIP(hipMemcpyAsync(s.h_hits, s.buf.hit.p, size_t(nh) * 64, hipMemcpyDeviceToHost, s.stream));
+        OXR_HIP(hipStreamSynchronize(s.stream));
+    }
+    unsigned long long ctr[8] = {0};
+    OXR_HIP(hipMemcpyAsync(ctr, s.buf.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s.stream));
+    OXR_HIP(hipStreamSynchronize(s.stream));
     if (ctr[4] != 0) return fail(OXR_EINVAL, "traversal stack overflow");
-    long long nh = offs[size_t(nb)];
-    std::vector<float> hbuf(size_t(nh) * 16);
-    if (nh > 0) OXR_HIP(hipMemcpy(hbuf.data(), step.hit.p, size_t(nh) * 64, hipMemcpyDeviceToHost));
     float kms = 0.f;
-    (void)hipEventElapsedTime(&kms, e_ev0, e_ev1);
-    for (size_t k = 0; k < segs.size(); k++) {
-        long long h0 = offs[size_t(segs[k].start / CH)];
-        long long h1 = k + 1 < segs.size() ? offs[size_t(segs[k + 1].start / CH)] : nh;
-        Req* r = owner[k];
-        r->hits.insert(r->hits.end(), hbuf.begin() + h0 * 16, hbuf.begin() + h1 * 16);
+    (void)hipEventElapsedTime(&kms, s.e0, s.e1);
+    for (size_t k = 0; k < s.segs.size(); k++) {
+        long long h0 = s.h_offs[size_t(s.segs[k].start / CH)];
+        long long h1 = k + 1 < s.segs.size() ? s.h_offs[size_t(s.segs[k + 1].start / CH)] : nh;
+        Req* r = s.owner[k];
+        if (h1 > h0) r->hits.insert(r->hits.end(), s.h_hits + h0 * 16, s.h_hits + h1 * 16);
         r->nhit += h1 - h0;
-        r->dispatched += take[k];
     }
     auto w1 = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> lk(mu);
     stat_steps += 1;
-    for (auto t : take) stat_photons += double(t);
-    stat_busy_ms += std::chrono::duration<double, std::milli>(w1 - w0).count();
+    for (auto t : s.take) stat_photons += double(t);
+    stat_busy_ms += std::chrono::duration<double, std::milli>(w1 - s.t0).count();
     stat_kernel_ms += kms;
     stat_bounce += double(ctr[0]);
     stat_hits += double(nh);
@@ -169,13 +226,14 @@ void Server::loop() {
         std::vector<std::unique_ptr<Req>> admitted;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !incoming.empty() || !active.empty(); });
+            bool inflight = slot[0].busy || slot[1].busy;
+            cv_work.wait(lk, [&] { return stop || !incoming.empty() || !active.empty() || inflight; });
             if (stop) break;
             while (!incoming.empty()) {
                 Req& r = *incoming.front();
-                bool slot = int(active.size() + admitted.size()) < cfg.max_active;
+                bool room = int(active.size() + admitted.size()) < cfg.max_active;
                 bool mem = used + r.bytes <= budget || (active.empty() && admitted.empty());
-                if (!slot || !mem) break;
+                if (!room || !mem) break;
                 used += r.bytes;
                 r.t_admit = clock();
                 admitted.push_back(std::move(incoming.front()));
@@ -187,16 +245,27 @@ void Server::loop() {
             if (rc != OXR_OK || r->num_photon == 0) complete(r, rc);
             else active.push_back(std::move(r));
         }
-        if (active.empty()) continue;
-        int rc = run_step();
+        int rc = OXR_OK;
+        Slot* fr = !slot[0].busy ? &slot[0] : (!slot[1].busy ? &slot[1] : nullptr);
+        if (fr && dispatchable()) {
+            rc = launch_step(*fr);
+        } else if (slot[0].busy || slot[1].busy) {
+            Slot* old = !slot[0].busy ? &slot[1] : (!slot[1].busy ? &slot[0] : (slot[0].seqno < slot[1].seqno ? &slot[0] : &slot[1]));
+            rc = finalize_step(*old);
+        } else {
+            continue;
+        }
         if (rc != OXR_OK) {
+            for (auto& sl : slot) { if (sl.busy) { (void)hipStreamSynchronize(sl.stream); sl.busy = false; } }
             for (auto& r : active) complete(r, rc);
             active.clear();
             continue;
         }
-        for (auto& r : active) if (r->dispatched >= r->num_photon) complete(r, OXR_OK);
+        for (auto& r : active)
+            if (r->dispatched >= r->num_photon && r->inflight == 0) complete(r, OXR_OK);
         active.erase(std::remove_if(active.begin(), active.end(), [](const std::unique_ptr<Req>& r) { return !r; }), active.end());
     }
+    for (auto& sl : slot) if (sl.busy) { (void)hipStreamSynchronize(sl.stream); sl.busy = false; }
     for (auto& r : active) complete(r, OXR_ESTATE);
     active.clear();
 }
@@ -218,13 +287,27 @@ int oxr_server_create(oxr_engine* eh, const oxr_server_config* cfg, oxr_server**
     if (s->cfg.hit_fraction <= 0.f) s->cfg.hit_fraction = 0.25f;
     if (s->cfg.form != OXR_FORM_PLAIN) s->cfg.form = OXR_FORM_PERSISTENT;
     if (s->cfg.max_active <= 0) s->cfg.max_active = 256;
-    hipError_t he = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (he != hipSuccess) { delete s; return hip_fail(he, "server stream"); }
-    if ((he = hipEventCreate(&s->e_ev0)) != hipSuccess || (he = hipEventCreate(&s->e_ev1)) != hipSuccess) {
-        (void)hipStreamDestroy(s->stream); delete s; return hip_fail(he, "server events");
+    auto bail = [&](int rc) {
+        for (auto& sl : s->slot) {
+            sl.buf.release_all(); sl.d_segs.release();
+            if (sl.h_offs) (void)hipHostFree(sl.h_offs);
+            if (sl.h_hits) (void)hipHostFree(sl.h_hits);
+            if (sl.e0) (void)hipEventDestroy(sl.e0);
+            if (sl.e1) (void)hipEventDestroy(sl.e1);
+            if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        }
+        if (s->admit_stream) (void)hipStreamDestroy(s->admit_stream);
+        delete s;
+        return rc;
+    };
+    hipError_t he;
+    if ((he = hipStreamCreateWithFlags(&s->admit_stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "server stream"));
+    for (auto& sl : s->slot) {
+        if ((he = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "server stream"));
+        if ((he = hipEventCreate(&sl.e0)) != hipSuccess || (he = hipEventCreate(&sl.e1)) != hipSuccess) return bail(hip_fail(he, "server events"));
+        int rc = ensure_event_capacity(*e, sl.buf, s->cfg.max_step_photons + 4096 * 64);
+        if (rc) return bail(rc);
     }
-    int rc = ensure_event_capacity(*e, s->step, s->cfg.max_step_photons + 4096 * 64);
-    if (rc) { s->step.release_all(); (void)hipStreamDestroy(s->stream); delete s; return rc; }
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
     s->budget = s->cfg.hbm_budget_bytes > 0 ? s->cfg.hbm_budget_bytes : int64_t(0.8 * double(fr));
@@ -242,14 +325,18 @@ void oxr_server_destroy(oxr_server* sh) {
     }
     s->cv_work.notify_all();
     if (s->th.joinable()) s->th.join();
-    (void)hipStreamSynchronize(s->stream);
     for (auto& r : s->incoming) r->free_device();
     for (auto& kv : s->finished) kv.second->free_device();
-    s->step.release_all();
-    s->d_segs.release();
-    (void)hipEventDestroy(s->e_ev0);
-    (void)hipEventDestroy(s->e_ev1);
-    (void)hipStreamDestroy(s->stream);
+    for (auto& sl : s->slot) {
+        (void)hipStreamSynchronize(sl.stream);
+        sl.buf.release_all(); sl.d_segs.release();
+        if (sl.h_offs) (void)hipHostFree(sl.h_offs);
+        if (sl.h_hits) (void)hipHostFree(sl.h_hits);
+        (void)hipEventDestroy(sl.e0);
+        (void)hipEventDestroy(sl.e1);
+        (void)hipStreamDestroy(sl.stream);
+    }
+    (void)hipStreamDestroy(s->admit_stream);
     delete s;
 }
 

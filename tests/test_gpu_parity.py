@@ -70,6 +70,22 @@ def test_simulate_bitexact(engine_mod, oracle_lib, name, schedule):
 
 
 @pytest.mark.parametrize('name', list(SCENES))
+@pytest.mark.parametrize('form', [abi.OXR_FORM_PLAIN, abi.OXR_FORM_PERSISTENT])
+def test_production_kernels_bitexact(engine_mod, oracle_lib, name, form):
+    """Without seqhis/records/precooked streams the engine runs its lean kernel variants;
+    their photons and hits must equal the oracle's too."""
+    s = scene(name)
+    cfg = abi.default_config(**s.config)
+    gs = s.gensteps(scale_photons=SCENES[name][1])
+    g = run_engine(engine_mod, s, gs, cfg, form)
+    r = run_oracle(oracle_lib, s, gs, cfg, cull=True)
+    assert np.array_equal(bits(g['photon']), bits(r['photon']))
+    sel = (bits(r['photon'])[:, 3, 3] & cfg.hitmask) != 0
+    assert np.array_equal(bits(g['hit']), bits(r['photon'][sel]))
+    assert g['stats']['bounce_steps'] == int(r['bounce'].sum())
+
+
+@pytest.mark.parametrize('name', list(SCENES))
 def test_persistent_equals_plain(engine_mod, name):
     s = scene(name)
     cfg = abi.default_config(**s.config, seq_enabled=1)
