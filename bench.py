@@ -109,6 +109,81 @@ def cpu_baseline(scene, gs, cfg, budget_s=12.0):
                        f'{dt:.1f} s', bounce_steps_per_s=float(r['bounce'].sum()) / dt)
 
 
+def history_chi2_leg(engine_mod, scene, gs, cfg, sample, threads, tune=None, inp=None, begin=0):
+    """seqhis chi2/ndf of the GPU FP32 run (event 1) against the FP64 oracle (event 2) over
+    the same photon index range [begin, begin+sample): independent random streams, so
+    chi2/ndf near 1 means the two histories are samples of one distribution."""
+    import oracle as O
+    from oxr import abi, qcf
+    c = abi.default_config(**{k: getattr(cfg, k) for k, _ in abi.Config._fields_})
+    c.seq_enabled = 1
+    e = engine_mod.Engine(scene.geometry, scene.physics, c)
+    if tune:
+        e.set_tuning(**tune)
+    if inp is not None:
+        e.set_input_photons(inp)
+    e.set_gensteps(gs, event_index=1)
+    e.launch(abi.OXR_FORM_PERSISTENT)
+    ga = e.seq()[begin:begin + sample]
+    e.close()
+    o = O.Oracle(scene.geometry, scene.physics, c)
+    o.set_cull(True)
+    t0 = time.perf_counter()
+    r = o.simulate(gs, event_index=2, precision=1, nthreads=threads, begin=begin, end=begin + sample,
+                   input_photons=inp)
+    dt = time.perf_counter() - t0
+    res = qcf.history_chi2(qcf.seqhis(ga), qcf.seqhis(r['seq']))
+    res.update(sample=int(len(ga)), oracle_s=round(dt, 2))
+    return res
+
+
+def precision_leg(engine_mod, threads, sample=5000):
+    """Thin shells, 5M photons (10% grazing input photons): FP32 vs FP32+FP64 roots
+    throughput on the GPU, and both against the FP64 oracle on the same streams over a
+    sample of grazing and isotropic photons."""
+    import oracle as O
+    from oxr import abi, qcf, scenes, gensteps as G
+    s = scenes.thin_shells()
+    n_total = 5_000_000
+    ng = n_total // 10
+    inp = scenes.thin_shell_grazing_photons(s, ng)
+    gs = np.concatenate([G.input_photon(ng), s.gensteps(scale_photons=(n_total - ng) / 5e6)])
+    tune = load_tuning('thin_shells')
+    rows = {}
+    o = O.Oracle(s.geometry, s.physics, abi.default_config(**s.config, seq_enabled=1))
+    o.set_cull(True)
+    t0 = time.perf_counter()
+    rg = o.simulate(gs, precision=1, nthreads=threads, begin=0, end=sample, input_photons=inp)
+    ri = o.simulate(gs, precision=1, nthreads=threads, begin=ng, end=ng + sample, input_photons=inp)
+    oracle_s = time.perf_counter() - t0
+    for f64 in (0, 1):
+        cfg = abi.default_config(**s.config, fp64_roots=f64)
+        e = engine_mod.Engine(s.geometry, s.physics, cfg)
+        if tune:
+            e.set_tuning(**tune)
+        e.set_input_photons(inp)
+        n = e.set_gensteps(gs)
+        dt, st, kms = timed_launches(e, abi.OXR_FORM_PERSISTENT, 3, 1)
+        e.set_config(seq_enabled=1)
+        e.set_gensteps(gs)
+        e.launch(abi.OXR_FORM_PERSISTENT)
+        ph, sq = e.photons(), e.seq()
+        e.close()
+        gra = qcf.endpoint_errors(ph[:sample], sq[:sample], rg['photon'], rg['seq'])
+        iso = qcf.endpoint_errors(ph[ng:ng + sample], sq[ng:ng + sample], ri['photon'], ri['seq'])
+        rows['fp64_roots' if f64 else 'fp32'] = {
+            'photons_per_s': round(3 * n / dt, 1), 'kernel_ms': round(kms, 3),
+            'bounce_steps': int(st['bounce_steps']),
+            'grazing_vs_fp64': {k: (round(v, 6) if isinstance(v, float) else v) for k, v in gra.items()},
+            'isotropic_vs_fp64': {k: (round(v, 6) if isinstance(v, float) else v) for k, v in iso.items()}}
+    rows['fp64_roots_cost'] = round(rows['fp32']['photons_per_s'] / rows['fp64_roots']['photons_per_s'], 4)
+    rows['workload'] = ('thin_shells: 50 x 1 mm shells at 1-20 m (n 1.50/1.33) + 5000 instanced zsphere|cylinder; '
+                        f'{n_total} photons at 400 nm, {ng} grazing (within 1e-3 rad of tangent) input photons '
+                        'then isotropic torch from the origin; max_bounce 64')
+    rows['oracle_sample'] = f'{sample} grazing + {sample} isotropic photons, FP64 oracle, {threads} threads, {oracle_s:.1f} s'
+    return rows, (s, gs, inp, ng)
+
+
 def serving_events(rng, n_events, median, sigma, lo, hi, ext):
     counts = np.clip(np.exp(np.log(median) + sigma * rng.normal(size=n_events)), lo, hi).astype(np.int64)
     from oxr import gensteps as G
@@ -180,6 +255,7 @@ def main():
     ap.add_argument('--quick', action='store_true', help='skip serving, CPU baseline and instanced-CSG legs')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
+    ap.add_argument('--no-precision', action='store_true', help='skip the thin-shell precision and history-chi2 legs')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one MI355X: this engine runs one process on one GPU and the '
@@ -263,6 +339,15 @@ def main():
         }
         log(f"[bench] serving: {out['serving']}")
     e.close()
+    threads = min(16, os.cpu_count() or 1)
+    do_prec = not (args.quick or args.no_precision)
+    chi2 = {}
+    if do_prec:
+        chi2['csi_calorimeter'] = history_chi2_leg(engine_mod, s, gs, cfg, 50_000, threads, tune=tune)
+        s1 = scenes.nested_boxes()
+        chi2['nested_boxes'] = history_chi2_leg(engine_mod, s1, s1.gensteps(), abi.default_config(**s1.config),
+                                                100_000, threads)
+        log(f"[bench] history chi2: {chi2}")
 
     if not args.quick:
         # ------------------------------------------------------------ instanced CSG
@@ -297,6 +382,18 @@ def main():
         }
         log(f"[bench] instanced_csg: {out['instanced_csg']}")
         e3.close()
+        if do_prec:
+            chi2['instanced_csg'] = history_chi2_leg(engine_mod, s3, gs3, cfg3, 20_000, threads, tune=t3)
+    if do_prec:
+        prec, (s5, gs5, inp5, ng5) = precision_leg(engine_mod, threads)
+        out['thin_shells_precision'] = prec
+        log(f"[bench] thin shells precision: {prec}")
+        chi2['thin_shells_isotropic'] = history_chi2_leg(engine_mod, s5, gs5, abi.default_config(**s5.config),
+                                                         20_000, threads, tune=load_tuning('thin_shells'),
+                                                         inp=inp5, begin=ng5)
+        out['history_chi2'] = {'definition': 'seqhis chi2/ndf, GPU FP32 (event 1) vs FP64 oracle (event 2), '
+                                              'histories with a+b > 30 (c2per = c2sum/c2n)', **chi2}
+        log(f"[bench] history chi2: {out['history_chi2']}")
     print(json.dumps(out), flush=True)
 
 
