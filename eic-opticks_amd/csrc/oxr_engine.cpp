@@ -24,7 +24,7 @@ int hip_fail(hipError_t e, const char* what) {
 void EventBuffers::release_all() {
     gs.release(); gs_off.release(); seed.release(); photon.release(); seq.release(); record.release();
     hitflag.release(); hit.release(); counts.release(); offs.release(); counters.release();
-    input.release(); pre.release();
+    input.release(); pre.release(); lite.release(); hitlite.release(); merged.release();
 }
 
 static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
@@ -59,6 +59,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.seq = cfg.seq_enabled ? b.seq.p : nullptr;
     P.record = cfg.max_record > 0 ? b.record.p : nullptr;
     P.max_record = cfg.max_record;
+    P.lite = cfg.photonlite ? b.lite.p : nullptr;
     P.hitflag = b.hitflag.p;
     P.bounce_out = nullptr; P.overflow_out = nullptr;
     P.counters = b.counters.p;
@@ -84,6 +85,7 @@ int ensure_event_capacity(Engine& e, EventBuffers& b, int64_t n) {
     OXR_HIP(b.photon.reserve(4 * N));
     if (e.cfg.seq_enabled) OXR_HIP(b.seq.reserve(2 * N));
     if (e.cfg.max_record > 0) OXR_HIP(b.record.reserve(4 * N * size_t(e.cfg.max_record)));
+    if (e.cfg.photonlite) OXR_HIP(b.lite.reserve(N));
     OXR_HIP(b.hitflag.reserve(N));
     OXR_HIP(b.hit.reserve(4 * N));
     int nb = oxl::compact_blocks(int64_t(N));
@@ -182,6 +184,7 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
         st->instance_visits = ctr[6];
     }
     e.num_hit = nhit;
+    e.hitlite_ready = false;
     return OXR_OK;
 }
 
@@ -277,6 +280,7 @@ void oxr_config_defaults(oxr_config* c) {
     c->rng_schedule = OXR_RNG_PRODUCTION;
     c->fp64_roots = 0; c->seq_enabled = 0; c->max_record = 0;
     c->max_photon = int64_t(1) << 24;
+    c->photonlite = 0; c->reserved = 0;
 }
 
 int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_config* cfg, oxr_engine** out) {
@@ -595,6 +599,21 @@ int oxr_event_get(oxr_engine* eh, int32_t what, void* dst, int64_t cap, int64_t*
             src = e->ev.record.p; n = e->num_photon * e->cfg.max_record; item = 64; break;
         case OXR_GET_SEED: src = e->ev.seed.p; n = e->num_photon; item = 4; break;
         case OXR_GET_GENSTEP: src = e->ev.gs.p; n = e->num_gs; item = 96; break;
+        case OXR_GET_PHOTONLITE:
+            if (!e->cfg.photonlite) return fail(OXR_ESTATE, "photonlite is off");
+            src = e->ev.lite.p; n = e->num_photon; item = 16; break;
+        case OXR_GET_HITLITE:
+            if (!e->cfg.photonlite) return fail(OXR_ESTATE, "photonlite is off");
+            if (e->launched && !e->hitlite_ready) {
+                // same selection and order as the sphoton hits (sphotonlite_selector,
+                // sphotonlite.h:159-165): reuse the launch's compaction offsets
+                OXR_HIP(e->ev.hitlite.reserve(size_t(std::max<int64_t>(e->num_hit, 1))));
+                OXR_HIP(oxl::scatter_lite(e->stream, e->ev.hitflag.p, e->ev.lite.p, e->num_photon, e->ev.offs.p,
+                                          e->ev.hitlite.p, e->num_hit));
+                OXR_HIP(hipStreamSynchronize(e->stream));
+                e->hitlite_ready = true;
+            }
+            src = e->ev.hitlite.p; n = e->num_hit; item = 16; break;
         default: return fail(OXR_EINVAL, "unknown gather kind");
     }
     if (what != OXR_GET_SEED && what != OXR_GET_GENSTEP && !e->launched) return fail(OXR_ESTATE, "no launch for this event");
@@ -605,6 +624,55 @@ int oxr_event_get(oxr_engine* eh, int32_t what, void* dst, int64_t cap, int64_t*
         OXR_HIP(hipMemcpy(dst, src, size_t(m) * item, hipMemcpyDeviceToHost));
     }
     return m < n ? fail(OXR_ECAPACITY, "destination smaller than count") : OXR_OK;
+}
+
+static int copy_out(const void* src, int64_t n, size_t item, void* dst, int64_t cap, int64_t* count) {
+    *count = n;
+    int64_t m = std::min(n, std::max<int64_t>(cap, 0));
+    if (m > 0) {
+        if (!dst) return fail(OXR_EINVAL, "null destination");
+        OXR_HIP(hipMemcpy(dst, src, size_t(m) * item, hipMemcpyDeviceToHost));
+    }
+    return m < n ? fail(OXR_ECAPACITY, "destination smaller than count") : OXR_OK;
+}
+
+int oxr_event_get_merged(oxr_engine* eh, int32_t what, float time_window, void* dst, int64_t cap, int64_t* count) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !count) return fail(OXR_EINVAL, "null argument");
+    if (what != OXR_GET_HIT && what != OXR_GET_HITLITE) return fail(OXR_EINVAL, "merge needs OXR_GET_HIT or OXR_GET_HITLITE");
+    if (!(time_window >= 0.f)) return fail(OXR_EINVAL, "time_window must be >= 0");
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (!e->launched) return fail(OXR_ESTATE, "no launch for this event");
+    bool lite = what == OXR_GET_HITLITE;
+    if (lite && !e->cfg.photonlite) return fail(OXR_ESTATE, "photonlite is off");
+    size_t words = lite ? 4 : 16;
+    const void* in = lite ? static_cast<const void*>(e->ev.lite.p) : static_cast<const void*>(e->ev.photon.p);
+    OXR_HIP(e->ev.merged.reserve(words * size_t(std::max<int64_t>(e->num_photon, 1))));
+    long long nout = 0;
+    OXR_HIP(oxl::merge(e->stream, lite, in, e->num_photon, e->cfg.hitmask, true, time_window, e->ev.merged.p, &nout));
+    return copy_out(e->ev.merged.p, nout, words * 4, dst, cap, count);
+}
+
+int oxr_merge_hits(oxr_engine* eh, int32_t what, float time_window, const void* in, int64_t n, void* dst, int64_t cap,
+                   int64_t* count) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !count || n < 0 || (n > 0 && !in)) return fail(OXR_EINVAL, "bad argument");
+    if (what != OXR_GET_HIT && what != OXR_GET_HITLITE) return fail(OXR_EINVAL, "merge needs OXR_GET_HIT or OXR_GET_HITLITE");
+    if (!(time_window >= 0.f)) return fail(OXR_EINVAL, "time_window must be >= 0");
+    std::lock_guard<std::mutex> lk(e->mu);
+    bool lite = what == OXR_GET_HITLITE;
+    size_t words = lite ? 4 : 16;
+    if (n == 0) { *count = 0; return OXR_OK; }
+    DBuf<uint32_t> din, dout;
+    auto done = [&](int rc) { din.release(); dout.release(); return rc; };
+    hipError_t he;
+    if ((he = din.reserve(words * size_t(n))) != hipSuccess) return done(hip_fail(he, "merge input"));
+    if ((he = dout.reserve(words * size_t(n))) != hipSuccess) return done(hip_fail(he, "merge output"));
+    if ((he = hipMemcpy(din.p, in, words * 4 * size_t(n), hipMemcpyHostToDevice)) != hipSuccess) return done(hip_fail(he, "merge upload"));
+    long long nout = 0;
+    if ((he = oxl::merge(e->stream, lite, din.p, n, 0u, false, time_window, dout.p, &nout)) != hipSuccess)
+        return done(hip_fail(he, "merge"));
+    return done(copy_out(dout.p, nout, words * 4, dst, cap, count));
 }
 
 int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_index, int32_t form, void* hits,

@@ -49,6 +49,9 @@ struct EventBuffers {
     DBuf<unsigned long long> counters;
     DBuf<float4> input;
     DBuf<float> pre;
+    DBuf<uint4> lite;          // sphotonlite per photon (config photonlite)
+    DBuf<uint4> hitlite;       // selected sphotonlite, formed on demand
+    DBuf<uint32_t> merged;     // merge output (sphoton or sphotonlite words)
     void release_all();
 };
 
@@ -85,6 +88,7 @@ struct Engine {
     int64_t pre_rows = 0;
     int num_bnd = 0;
     bool launched = false;
+    bool hitlite_ready = false;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
     std::mutex mu;   // serialises C-ABI calls on one engine
 

@@ -33,7 +33,8 @@ struct Params {
     ulonglong2* seq;           // sseq out (2 x ulonglong2 per photon) or null
     float4* record;            // records or null
     int max_record;
-    uint8_t* hitflag;          // 1 if flagmask & hitmask
+    uint4* lite;               // sphotonlite out (16 B per photon) or null
+    uint8_t* hitflag;          // 1 if (flagmask & hitmask) == hitmask
     uint32_t* bounce_out;      // optional per-photon loop count
     uint32_t* overflow_out;    // optional precooked-stream wrap count
     unsigned long long* counters;   // [0] steps [1] queue [2] node visits [3] prim tests [4] errors
@@ -64,6 +65,12 @@ hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int 
 int compact_blocks(long long n);
 hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
                    long long* offs, float4* hit, long long hit_cap);
+// hit selection of the sphotonlite array with the offsets a compact() of the same flags left
+hipError_t scatter_lite(hipStream_t s, const uint8_t* flag, const uint4* lite, long long n, const long long* offs,
+                        uint4* hitlite, long long hit_cap);
+// merge of sphoton (lite=false) or sphotonlite items by (identity, time bucket); see oxr_merge.hip
+hipError_t merge(hipStream_t s, bool lite, const void* in, long long n, unsigned select_mask, bool select,
+                 float time_window, void* out, long long* num_out);
 hipError_t trace_rays(hipStream_t s, const oxd::Params& P, const float4* rays, long long n, float4* prd, bool fp64);
 hipError_t intersect_prim(hipStream_t s, const oxd::Geom& g, int root, const float4* rays, long long n,
                           float4* isect, int* valid, bool fp64);

@@ -37,6 +37,7 @@ struct Lane {
     Ph p;
     Rng r;
     unsigned long long sh0, sh1, sb0, sb1;   // seqhis / seqbnd
+    float lc, lf;                            // lposcost, lposfphi of the last trace (photonlite)
     long long idx;
     int bounce;
 };
@@ -92,6 +93,7 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
     else { L.p = Ph{}; zero_flags(L.p); }
     set_index(L.p, pidx);
     L.sh0 = L.sh1 = L.sb0 = L.sb1 = 0ull;
+    if (DBG) { L.lc = 0.f; L.lf = 0.f; }
     L.bounce = 0;
     point<DBG>(P, L);
     return true;
@@ -106,9 +108,16 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
     Hit h;
     bool found = trace<FP64, COUNT>(P.g, L.p.pos, L.p.mom, tmin, P.tmax, h, stk, stride, P.stack_cap, top, ntop,
                                     visits, tests, err, csgb, ivis);
-    if (!found) return false;
+    if (!found) {
+        if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0
+        return false;
+    }
     f3 normal = normalize(h.n);
     float lposcost = h.lpos.z / sqrtf(dot(h.lpos, h.lpos));
+    if (DBG && P.lite) {
+        L.lc = lposcost;
+        L.lf = (xatan2(h.lpos.y, h.lpos.x) + PIf) / (2.0f * PIf);   // normalize_fphi, sysrap/scuda.h:653
+    }
     int command = propagate(L.p, h, normal, lposcost, L.r, P.t, P.aligned);
     L.bounce++;
     point<DBG>(P, L);
@@ -123,7 +132,9 @@ OXD void end_photon(const Params& P, const Lane& L) {
         s[1] = make_ulonglong2(L.sb0, L.sb1);
     }
     store_photon(P.photon + 4 * L.idx, L.p);
-    if (P.hitflag) P.hitflag[L.idx] = (L.p.flagmask & P.hitmask) ? 1 : 0;
+    // all bits of the hit mask must be set (sphoton_selector, sysrap/sphoton.h:1219-1225)
+    if (P.hitflag) P.hitflag[L.idx] = (L.p.flagmask & P.hitmask) == P.hitmask ? 1 : 0;
+    if (DBG && P.lite) P.lite[L.idx] = lite_pack(L.p, L.lc, L.lf);
     if (DBG && P.bounce_out) P.bounce_out[L.idx] = uint32_t(L.bounce);
     if (DBG && P.overflow_out) P.overflow_out[L.idx] = L.r.overflow;
 }
@@ -269,6 +280,8 @@ __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ counts, i
     if (threadIdx.x == 0) out[nb] = carry;
 }
 
+// W float4 per item: 4 for sphoton, 1 for sphotonlite
+template <int W>
 __global__ void __launch_bounds__(256) k_scatter(const uint8_t* __restrict__ flag, const float4* __restrict__ photon,
                                                  long long n, const long long* __restrict__ offs,
                                                  float4* __restrict__ hit, long long hit_cap) {
@@ -289,9 +302,10 @@ __global__ void __launch_bounds__(256) k_scatter(const uint8_t* __restrict__ fla
         if (f) {
             long long dst = o + before + rank;
             if (dst < hit_cap) {
-                const float4* s = photon + 4 * i;
-                float4* d = hit + 4 * dst;
-                d[0] = s[0]; d[1] = s[1]; d[2] = s[2]; d[3] = s[3];
+                const float4* s = photon + W * i;
+                float4* d = hit + W * dst;
+#pragma unroll
+                for (int q = 0; q < W; q++) d[q] = s[q];
             }
         }
         o += tot;
@@ -354,7 +368,7 @@ size_t simulate_smem(const Params& P, int block) {
 
 // debug outputs (seqhis, records, precooked streams) select the DBG kernel variants
 static bool wants_debug(const Params& P) {
-    return P.seq || P.record || P.pre || P.bounce_out || P.overflow_out;
+    return P.seq || P.record || P.pre || P.bounce_out || P.overflow_out || P.lite;
 }
 
 hipError_t simulate_plain(hipStream_t s, const Params& P, int block, bool fp64, bool count) {
@@ -397,7 +411,16 @@ hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, lon
     int nb = compact_blocks(n);
     hipLaunchKernelGGL(k_count, dim3(nb), dim3(256), 0, s, flag, n, counts);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, counts, nb, offs);
-    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(256), 0, s, flag, photon, n, offs, hit, hit_cap);
+    hipLaunchKernelGGL(k_scatter<4>, dim3(nb), dim3(256), 0, s, flag, photon, n, offs, hit, hit_cap);
+    return hipGetLastError();
+}
+
+hipError_t scatter_lite(hipStream_t s, const uint8_t* flag, const uint4* lite, long long n, const long long* offs,
+                        uint4* hitlite, long long hit_cap) {
+    if (n <= 0) return hipSuccess;
+    int nb = compact_blocks(n);
+    hipLaunchKernelGGL(k_scatter<1>, dim3(nb), dim3(256), 0, s, flag, reinterpret_cast<const float4*>(lite), n, offs,
+                       reinterpret_cast<float4*>(hitlite), hit_cap);
     return hipGetLastError();
 }
 

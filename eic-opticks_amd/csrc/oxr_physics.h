@@ -45,6 +45,20 @@ OXD void store_photon(float4* dst, const Ph& p) {
     dst[2] = make_float4(p.pol.x, p.pol.y, p.pol.z, p.wavelength);
     dst[3] = make_float4(__uint_as_float(p.obf), __uint_as_float(p.identity), __uint_as_float(p.index), __uint_as_float(p.flagmask));
 }
+
+// sphotonlite (sysrap/sphotonlite.h:47-52,184-190,234-243): hitcount 1 | identity low 16 bits,
+// time, lposcost and lposfphi as 16-bit unorms, flagmask. set_lpos converts
+// v*0xffff+0.5 to 16 bits; that conversion saturates (negative and NaN give 0).
+OXD uint32_t unorm16(float v) {
+    float x = v * 65535.f + 0.5f;
+    if (!(x > 0.f)) return 0u;
+    if (x >= 65535.f) return 65535u;
+    return uint32_t(x);
+}
+OXD uint4 lite_pack(const Ph& p, float lposcost, float lposfphi) {
+    return make_uint4(0x10000u | (p.identity & 0xffffu), __float_as_uint(p.time),
+                      (unorm16(lposcost) << 16) | unorm16(lposfphi), p.flagmask);
+}
 OXD void load_photon(Ph& p, const float4* s) {
     float4 a = s[0], b = s[1], c = s[2], e = s[3];
     p.pos = mk(a.x, a.y, a.z); p.time = a.w;

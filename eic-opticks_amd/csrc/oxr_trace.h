@@ -58,23 +58,34 @@ OXD bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, f3 
     return n <= f && f >= 0.f && n <= tbest;
 }
 
+// world -> object frame of one instance (rows c0..c2 hold the world-to-object 3x4)
+OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
+    o = mk(in.c0.x * pos.x + in.c0.y * pos.y + in.c0.z * pos.z + in.c0.w,
+           in.c1.x * pos.x + in.c1.y * pos.y + in.c1.z * pos.z + in.c1.w,
+           in.c2.x * pos.x + in.c2.y * pos.y + in.c2.z * pos.z + in.c2.w);
+    d = mk(in.c0.x * mom.x + in.c0.y * mom.y + in.c0.z * mom.z,
+           in.c1.x * mom.x + in.c1.y * mom.y + in.c1.z * mom.z,
+           in.c2.x * mom.x + in.c2.y * mom.y + in.c2.z * mom.z);
+}
+
 // Closest hit in (tmin, tmax]; ties in t go to the smaller (instance, local prim) key.
 // stk points at this lane's LDS stack column (element k at stk[k*stride]).
+// During traversal only (t, instance, local prim, object-frame normal) of the best hit
+// stay live; its world normal, local position and identity words are formed afterwards
+// from the instance record, with the same arithmetic as at instance entry.
 template <bool FP64, bool COUNT>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
     float best = tmax;
-    uint64_t best_key = ~0ull;
+    int best_inst = 0, best_item = 0;
+    f3 bn = mk(0.f, 0.f, 0.f);
     bool found = false;
     int sp = 0;
     int cur = g.tlas_root;
     bool at_top = true;
     f3 o = pos, d = mom, inv = safe_inv(mom);
     int inst = 0;
-    float4 c0 = make_float4(1.f, 0.f, 0.f, 0.f), c1 = make_float4(0.f, 1.f, 0.f, 0.f), c2 = make_float4(0.f, 0.f, 1.f, 0.f);
-    bool ident = true;
-    uint32_t ii_id = 0;
     int primOffset = 0;
     for (;;) {
         if (cur >= 0) {
@@ -110,20 +121,12 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         }
         int item = ~cur;
         if (at_top) {
-            InstD in = g.inst[item];
+            const InstD in = g.inst[item];
             if (COUNT) ivis++;
             inst = item;
-            c0 = in.c0; c1 = in.c1; c2 = in.c2;
-            ident = (in.info.z & 1) != 0;
-            ii_id = uint32_t(in.info.y);
             primOffset = in.info.x;
-            if (!ident) {
-                o = mk(c0.x * pos.x + c0.y * pos.y + c0.z * pos.z + c0.w,
-                       c1.x * pos.x + c1.y * pos.y + c1.z * pos.z + c1.w,
-                       c2.x * pos.x + c2.y * pos.y + c2.z * pos.z + c2.w);
-                d = mk(c0.x * mom.x + c0.y * mom.y + c0.z * mom.z,
-                       c1.x * mom.x + c1.y * mom.y + c1.z * mom.z,
-                       c2.x * mom.x + c2.y * mom.y + c2.z * mom.z);
+            if ((in.info.z & 1) == 0) {
+                to_object(in, pos, mom, o, d);
                 inv = safe_inv(d);
             }
             if (sp < cap) { stk[sp * stride] = SENTINEL; sp++; } else { err |= 1u; }
@@ -132,35 +135,25 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             continue;
         }
         // prim leaf: item is the local prim index within the instance's solid
-        int2 pr = g.prim[primOffset + item];
+        int nodeOffset = g.prim[primOffset + item].x;
         if (COUNT) {
             tests++;
-            unsigned tc0 = __float_as_uint(g.node[4 * pr.x + 3].z);
-            unsigned nn = tc0 >= CSG_LEAF ? 1u : __float_as_uint(g.node[4 * pr.x].x);
+            unsigned tc0 = __float_as_uint(g.node[4 * nodeOffset + 3].z);
+            unsigned nn = tc0 >= CSG_LEAF ? 1u : __float_as_uint(g.node[4 * nodeOffset].x);
             for (unsigned k = 0; k < nn; k++) {
-                float4 q3 = g.node[4 * (pr.x + int(k)) + 3];
+                float4 q3 = g.node[4 * (nodeOffset + int(k)) + 3];
                 csgb += 64u;
                 if (__float_as_uint(q3.z) >= CSG_LEAF && (__float_as_uint(q3.w) & 0x7fffffffu) != 0u) csgb += 64u;
             }
         }
         Is is;
-        if (intersect_prim<FP64>(is, g.node, pr.x, g.plan, g.itra, tmin, o, d)) {
+        if (intersect_prim<FP64>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
             float t = is.w;
-            uint64_t key = (uint64_t(uint32_t(inst)) << 32) | uint64_t(uint32_t(item));
-            if (t < best || (found && t == best && key < best_key)) {
+            bool closer = t < best || (found && t == best && (inst < best_inst || (inst == best_inst && item < best_item)));
+            if (closer) {
                 found = true;
-                best = t; best_key = key;
-                hit.t = t;
-                hit.lpos = mk(o.x + t * d.x, o.y + t * d.y, o.z + t * d.z);
-                hit.gpbd = uint32_t(pr.y);
-                hit.ii_id = ii_id;
-                if (!ident) {
-                    hit.n = mk(c0.x * is.x + c1.x * is.y + c2.x * is.z,
-                               c0.y * is.x + c1.y * is.y + c2.y * is.z,
-                               c0.z * is.x + c1.z * is.y + c2.z * is.z);
-                } else {
-                    hit.n = mk(is.x, is.y, is.z);
-                }
+                best = t; best_inst = inst; best_item = item;
+                bn = mk(is.x, is.y, is.z);
             }
         }
         if (sp == 0) break;
@@ -169,8 +162,24 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
     if (!found) {
         hit.t = 0.f; hit.n = mk(0.f, 0.f, 0.f); hit.lpos = mk(0.f, 0.f, 0.f);
         hit.ii_id = 0xffffffffu; hit.gpbd = 0xffffffffu;
+        return false;
     }
-    return found;
+    const InstD in = g.inst[best_inst];
+    f3 bo = pos, bd = mom;
+    bool ident = (in.info.z & 1) != 0;
+    if (!ident) to_object(in, pos, mom, bo, bd);
+    hit.t = best;
+    hit.lpos = mk(bo.x + best * bd.x, bo.y + best * bd.y, bo.z + best * bd.z);
+    hit.ii_id = uint32_t(in.info.y);
+    hit.gpbd = uint32_t(g.prim[in.info.x + best_item].y);
+    if (!ident) {
+        hit.n = mk(in.c0.x * bn.x + in.c1.x * bn.y + in.c2.x * bn.z,
+                   in.c0.y * bn.x + in.c1.y * bn.y + in.c2.y * bn.z,
+                   in.c0.z * bn.x + in.c1.z * bn.y + in.c2.z * bn.z);
+    } else {
+        hit.n = bn;
+    }
+    return true;
 }
 
 } // namespace oxd

@@ -57,6 +57,8 @@ def lib():
             'oxr_simulate_launch': (i32, [vp, i32, P(abi.LaunchStats)]),
             'oxr_simulate_launch_counting': (i32, [vp, P(abi.LaunchStats)]),
             'oxr_event_get': (i32, [vp, i32, vp, i64, P(i64)]),
+            'oxr_event_get_merged': (i32, [vp, i32, C.c_float, vp, i64, P(i64)]),
+            'oxr_merge_hits': (i32, [vp, i32, C.c_float, vp, i64, vp, i64, P(i64)]),
             'oxr_simulate': (i32, [vp, vp, i64, i32, i32, vp, i64, P(i64), P(abi.LaunchStats)]),
             'oxr_trace_rays': (i32, [vp, vp, i64, vp]),
             'oxr_intersect_prim': (i32, [vp, i32, vp, i64, vp, vp]),
@@ -183,6 +185,8 @@ class Engine:
             abi.OXR_GET_RECORD: ((n.value, 4, 4), np.float32),
             abi.OXR_GET_SEED: ((n.value,), np.int32),
             abi.OXR_GET_GENSTEP: ((n.value, 6, 4), np.float32),
+            abi.OXR_GET_PHOTONLITE: ((n.value, 4), np.uint32),
+            abi.OXR_GET_HITLITE: ((n.value, 4), np.uint32),
         }[what]
         out = np.zeros(shape, dtype=dt)
         m = C.c_int64()
@@ -193,6 +197,27 @@ class Engine:
 
     def photons(self):
         return self.get(abi.OXR_GET_PHOTON)
+
+    def merged(self, what=abi.OXR_GET_HIT, time_window=1.0):
+        """Hits of the last launch merged by (identity, time bucket) on the device."""
+        n = C.c_int64()
+        rc = lib().oxr_event_get_merged(self.h, what, C.c_float(time_window), None, 0, C.byref(n))
+        if rc not in (0, -4):
+            _check(rc)
+        shape = (n.value, 4, 4) if what == abi.OXR_GET_HIT else (n.value, 4)
+        out = np.zeros(shape, dtype=np.float32 if what == abi.OXR_GET_HIT else np.uint32)
+        m = C.c_int64()
+        _check(lib().oxr_event_get_merged(self.h, what, C.c_float(time_window), abi.ptr(out), n.value, C.byref(m)))
+        return out
+
+    def merge_hits(self, items, what=abi.OXR_GET_HIT, time_window=1.0):
+        """Merge an already selected host array (sphoton (n,4,4) or sphotonlite (n,4))."""
+        a = np.ascontiguousarray(items)
+        out = np.zeros_like(a)
+        m = C.c_int64()
+        _check(lib().oxr_merge_hits(self.h, what, C.c_float(time_window), abi.ptr(a), len(a), abi.ptr(out), len(a),
+                                    C.byref(m)))
+        return out[:m.value]
 
     def hits(self):
         return self.get(abi.OXR_GET_HIT)

@@ -70,7 +70,11 @@ enum {
     OXR_GET_SEQ    = 2,   /* sseq[num_photon] (needs seq_enabled) QEvt::gatherSeq    */
     OXR_GET_RECORD = 3,   /* sphoton[num_photon*max_record]      QEvt::gatherRecord  */
     OXR_GET_SEED   = 4,   /* int32[num_photon] genstep index     QEvt seed buffer    */
-    OXR_GET_GENSTEP= 5    /* quad6[num_genstep]                                      */
+    OXR_GET_GENSTEP= 5,   /* quad6[num_genstep]                                      */
+    OXR_GET_PHOTONLITE = 6, /* sphotonlite[num_photon] (needs photonlite)
+                               QEvt::gatherPhotonLite, sysrap/sphotonlite.h:47-52     */
+    OXR_GET_HITLITE    = 7  /* sphotonlite[num_hit]: the photonlite items of the hits,
+                               QEvt::gatherHitLite (qudarap/QEvt.cc:985-1019)         */
 };
 
 /* ---- inputs ----------------------------------------------------------- */
@@ -121,6 +125,10 @@ typedef struct oxr_config {
     int32_t  seq_enabled;     /* 1: write sseq seqhis/seqbnd per photon               */
     int32_t  max_record;      /* >0: write per-bounce sphoton records                 */
     int64_t  max_photon;      /* capacity of one launch slice (OPTICKS_MAX_SLOT)      */
+    int32_t  photonlite;      /* 1: also write sphotonlite (16 B) per photon, the last
+                                 intersect's lposcost/lposfphi packed (SEventConfig
+                                 ModeLite, CSGOptiX7.cu:493-499); default 0           */
+    int32_t  reserved;        /* 0                                                    */
 } oxr_config;
 
 /* Persistent-form launch parameters, chosen by the autotuner. */
@@ -210,6 +218,23 @@ int oxr_simulate_launch_counting(oxr_engine* e, oxr_launch_stats* stats);
  * (qudarap/QEvt.cc:934-970,1113-1133). Copies min(count, capacity) items to host. */
 int oxr_event_get(oxr_engine* e, int32_t what, void* dst, int64_t capacity,
                   int64_t* count);
+
+/* Hit merging on the device: replaces QEvt::gatherHitMerged / gatherHitLiteMerged
+ * (qudarap/QEvt.cc:1034-1086,1159-1195) and SPM::merge_partial_select (sysrap/SPM.cu:135-260).
+ * The current launch's photons (what = OXR_GET_HIT, sphoton) or photonlite items
+ * (what = OXR_GET_HITLITE, needs photonlite) whose flagmask shares any bit with the
+ * config hitmask are keyed (identity<<48 | unsigned(time/time_window)), stably sorted by
+ * key, and each run of equal keys is folded into one item: flagmasks OR-ed, 16-bit
+ * hitcounts summed, sphoton keeping the earliest item, sphotonlite the first item with
+ * the run's minimum time (reduce_op, sphoton.h:288-298, sphotonlite.h:79-90). Items come
+ * out in key order. time_window 0 returns the selection in photon order, unmerged. */
+int oxr_event_get_merged(oxr_engine* e, int32_t what, float time_window, void* dst,
+                         int64_t capacity, int64_t* count);
+
+/* Merge an already selected host array, e.g. the concatenated per-slice merges of a sliced
+ * event: replaces QEvt::FinalMerge (qudarap/QEvt.cc:1236-1300, ALREADY_HITMASK_SELECTED). */
+int oxr_merge_hits(oxr_engine* e, int32_t what, float time_window, const void* items,
+                   int64_t n, void* dst, int64_t capacity, int64_t* count);
 
 /* One request, one event: replaces CSGOptiXService::simulate(gs, eventID) -> hits
  * (CSGOptiX/CSGOptiXService.h:76-84, QSim::simulate qudarap/QSim.cc:711-740). Slices

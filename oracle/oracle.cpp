@@ -1316,7 +1316,17 @@ struct Sim {
 struct Outputs {
     float* photon; uint64_t* seq; float* record; int32_t max_record;
     uint32_t* bounce; uint32_t* overflow;
+    uint32_t* lite;     // sphotonlite (4 words) per photon or null
 };
+
+// sphotonlite::set_lpos (sysrap/sphotonlite.h:234-243): v*0xffff+0.5 converted to 16 bits;
+// the conversion saturates, negative and NaN giving 0.
+inline uint32_t lite_unorm16(float v) {
+    float x = v * 65535.f + 0.5f;
+    if (!(x > 0.f)) return 0u;
+    if (x >= 65535.f) return 65535u;
+    return uint32_t(x);
+}
 
 // simulate (CSGOptiX/CSGOptiX7.cu:405-503) for one photon
 template <typename T>
@@ -1353,11 +1363,13 @@ void simulate_one(const Ctx& c, const float* gs_all, const int32_t* seed, int64_
         }
     };
     int bounce = 0;
+    float last_lposcost = 0.f, last_lposfphi = 0.f;   // PRD of the last trace (zero on a miss)
     point(bounce);
     while (bounce < c.cfg.max_bounce && p.time < T(c.cfg.max_time)) {
         T tmin = (p.orient_boundary_flag & c.cfg.eps0_mask) ? T(c.cfg.tmin0) : T(c.cfg.tmin);
         PRD<T> prd;
         trace<T>(c, p.pos, p.mom, tmin, T(c.cfg.tmax), prd);
+        last_lposcost = prd.lposcost; last_lposfphi = prd.lposfphi;
         if ((prd.gp_boundary & 0xffffu) == 0xffffu) break;
         V3<T> normal = normalize(prd.normal);
         int command = sim.propagate(p, prd, normal);
@@ -1370,6 +1382,14 @@ void simulate_one(const Ctx& c, const float* gs_all, const int32_t* seed, int64_
         s[0] = seqhis[0]; s[1] = seqhis[1]; s[2] = seqbnd[0]; s[3] = seqbnd[1];
     }
     p.store(out.photon + 16 * size_t(idx));
+    if (out.lite) {
+        // CSGOptiX7.cu:493-499: sphotonlite::init(identity, time, flagmask) + set_lpos
+        uint32_t* l = out.lite + 4 * size_t(idx);
+        l[0] = 0x10000u | (p.identity & 0xffffu);
+        l[1] = f2u(float(p.time));
+        l[2] = (lite_unorm16(last_lposcost) << 16) | lite_unorm16(last_lposfphi);
+        l[3] = p.flagmask;
+    }
     if (out.bounce) out.bounce[idx] = uint32_t(bounce);
     if (out.overflow) out.overflow[idx] = sim.rng.overflow;
 }
@@ -1506,14 +1526,14 @@ int64_t orc_simulate(void* cv, const float* gs, int64_t num_gs, int32_t event_in
                      const float* precooked, int32_t npre, const float* input_photons,
                      int32_t precision, int32_t nthreads, int64_t photon_begin, int64_t photon_end,
                      float* photon, uint64_t* seq, float* record, int32_t max_record, uint32_t* bounce,
-                     uint32_t* overflow) {
+                     uint32_t* overflow, uint32_t* lite) {
     const orc::Ctx& c = *static_cast<orc::Ctx*>(cv);
     int64_t total = orc_fill_seed(gs, num_gs, nullptr, 0);
     std::vector<int32_t> seed(static_cast<size_t>(total));
     orc_fill_seed(gs, num_gs, seed.data(), total);
     if (photon_end < 0 || photon_end > total) photon_end = total;
     if (photon_begin < 0) photon_begin = 0;
-    orc::Outputs out{photon, seq, record, max_record, bounce, overflow};
+    orc::Outputs out{photon, seq, record, max_record, bounce, overflow, lite};
     std::atomic<int64_t> next(photon_begin);
     auto work = [&]() {
         for (;;) {

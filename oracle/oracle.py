@@ -44,7 +44,7 @@ def lib():
         L.orc_trace_rays.argtypes = [vp, vp, i64, vp, i32]
         L.orc_simulate.restype = i64
         L.orc_simulate.argtypes = [vp, vp, i64, i32, u64, vp, i32, vp, i32, i32, i64, i64,
-                                   vp, vp, vp, i32, vp, vp]
+                                   vp, vp, vp, i32, vp, vp, vp]
         L.orc_propagate_at_boundary.argtypes = [vp, C.c_float, C.c_float, vp, i32, i32, vp]
         L.orc_sample.argtypes = [i32, vp, i64, u64, vp]
         L.orc_boundary_lookup.argtypes = [vp, vp, i64, u32, u32, vp]
@@ -101,6 +101,63 @@ def sample(which, mom, pol, normal, n, seed=1):
     return out[:, 0:3], out[:, 3:6]
 
 
+def time_bucket(time, tw):
+    """unsigned(time / tw) in float32 with the saturating float->unsigned conversion."""
+    q = (np.asarray(time, dtype=np.float32) / np.float32(tw)).astype(np.float32)
+    out = np.zeros(q.shape, dtype=np.uint64)
+    ok = q > 0
+    big = q >= np.float32(4294967296.0)
+    out[ok & ~big] = q[ok & ~big].astype(np.uint64)
+    out[big] = 0xffffffff
+    return out
+
+
+def merge(items, time_window, mask=None):
+    """SPM::merge_partial_select (sysrap/SPM.cu:135-260) restated with numpy loops.
+
+    items: sphoton (n,4,4) f32 / (n,16) or sphotonlite (n,4) u32. mask: select items whose
+    flagmask shares a bit with it (select_pred, sphoton.h:300-304, sphotonlite.h:93-97);
+    None = already selected. Key (identity<<48 | unsigned(time/tw)) in 64-bit arithmetic
+    (key_functor, sphoton.h:277-286, sphotonlite.h:68-77); stable sort by key; each run
+    folded left to right with reduce_op (sphoton.h:288-298, sphotonlite.h:79-90)."""
+    u = np.ascontiguousarray(items).view(np.uint32)
+    u = u.reshape(len(u), -1).copy()
+    lite = u.shape[1] == 4
+    TIME, FM = (1, 3) if lite else (3, 15)
+    if mask is not None:
+        u = u[(u[:, FM] & np.uint32(mask)) != 0]
+    if time_window == 0 or len(u) == 0:
+        return u
+    ident = (u[:, 0] & 0xffff) if lite else (u[:, 13] & 0x00ffffff)
+    t = u[:, TIME].view(np.float32)
+    key = (ident.astype(np.uint64) << np.uint64(48)) | time_bucket(t, time_window)
+    order = np.argsort(key, kind='stable')
+    out = []
+    i = 0
+    n = len(order)
+    while i < n:
+        j = i
+        while j + 1 < n and key[order[j + 1]] == key[order[i]]:
+            j += 1
+        r = u[order[i]].copy()
+        for k in range(i + 1, j + 1):
+            b = u[order[k]]
+            ra, rb = r[TIME:TIME + 1].view(np.float32)[0], b[TIME:TIME + 1].view(np.float32)[0]
+            hc = ((int(r[0 if lite else 7]) >> 16) + (int(b[0 if lite else 7]) >> 16)) & 0xffff
+            fm = int(r[FM]) | int(b[FM])
+            if lite:
+                r[TIME] = np.float32(np.fmin(ra, rb)).view(np.uint32)      # r = a; r.time = fminf
+                r[0] = (hc << 16) | (int(r[0]) & 0xffff)                   # set_hitcount_identity
+            else:
+                if not (np.fmin(ra, rb) == ra):                            # r = a_first ? a : b
+                    r = b.copy()
+                r[7] = (int(r[7]) & 0xffff) | (hc << 16)                   # set_hitcount
+            r[FM] = fm
+        out.append(r)
+        i = j + 1
+    return np.array(out, dtype=np.uint32)
+
+
 class Oracle:
     """Holds an oracle context over one scene's arrays."""
 
@@ -137,7 +194,7 @@ class Oracle:
         return prd
 
     def simulate(self, gs, event_index=0, slot_offset=0, precooked=None, input_photons=None,
-                 precision=0, nthreads=8, begin=0, end=-1, seq=True, max_record=0):
+                 precision=0, nthreads=8, begin=0, end=-1, seq=True, max_record=0, lite=False):
         gs = np.ascontiguousarray(gs, dtype=np.float32)
         total = len(fill_seed(gs))
         if end < 0 or end > total:
@@ -147,6 +204,7 @@ class Oracle:
         rec = np.zeros((total, max_record, 4, 4), dtype=np.float32) if max_record > 0 else None
         bounce = np.zeros(total, dtype=np.uint32)
         overflow = np.zeros(total, dtype=np.uint32)
+        lt = np.zeros((total, 4), dtype=np.uint32) if lite else None
         pre = None
         npre = 0
         if precooked is not None:
@@ -155,11 +213,11 @@ class Oracle:
         inp = None if input_photons is None else np.ascontiguousarray(input_photons, dtype=np.float32)
         lib().orc_simulate(self.h, P(gs), len(gs), event_index, slot_offset, P(pre), npre, P(inp),
                            precision, nthreads, begin, end, P(photon), P(sq), P(rec), max_record,
-                           P(bounce), P(overflow))
+                           P(bounce), P(overflow), P(lt))
         sl = slice(begin, end)
         return dict(photon=photon[sl], seq=None if sq is None else sq[sl],
                     record=None if rec is None else rec[sl], bounce=bounce[sl],
-                    overflow=overflow[sl], total=total)
+                    overflow=overflow[sl], total=total, lite=None if lt is None else lt[sl])
 
     def boundary_lookup(self, nm, line, k):
         nm = np.ascontiguousarray(nm, dtype=np.float32)

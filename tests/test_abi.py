@@ -26,7 +26,7 @@ def test_every_declared_symbol_is_exported(engine_mod):
 def test_struct_sizes_match_header(engine_mod):
     # sizes implied by include/oxr.h on LP64
     assert C.sizeof(abi.Geometry) == 96
-    assert C.sizeof(abi.Config) == 80
+    assert C.sizeof(abi.Config) == 88
     assert C.sizeof(abi.LaunchStats) == 80
     assert C.sizeof(abi.Tuning) == 16
     assert C.sizeof(abi.RequestResult) == 64

@@ -28,7 +28,7 @@ def _check_event(engine_mod, oracle_lib, s, gs, cfg, ranges):
     assert st['bounce_steps'] == st2['bounce_steps'] and st['num_hit'] == st2['num_hit']
     # hits are exactly the SD photons in order, each carrying its own photon index
     u = bits(p)
-    sel = (u[:, 3, 3] & cfg.hitmask) != 0
+    sel = (u[:, 3, 3] & cfg.hitmask) == cfg.hitmask
     assert np.array_equal(bits(h), u[sel])
     assert np.array_equal(u[:, 3, 2], np.arange(n, dtype=np.uint32))
     # every photon ended on a terminal flag or at the bounce limit

@@ -65,7 +65,7 @@ def test_simulate_bitexact(engine_mod, oracle_lib, name, schedule):
     assert np.array_equal(g['seq'], r['seq'])
     assert g['stats']['bounce_steps'] == int(r['bounce'].sum())
     # hits are the SD photons in photon order (SU::copy_if semantics)
-    sel = (bits(r['photon'])[:, 3, 3] & cfg.hitmask) != 0
+    sel = (bits(r['photon'])[:, 3, 3] & cfg.hitmask) == cfg.hitmask
     assert np.array_equal(bits(g['hit']), bits(r['photon'][sel]))
 
 
@@ -80,7 +80,7 @@ def test_production_kernels_bitexact(engine_mod, oracle_lib, name, form):
     g = run_engine(engine_mod, s, gs, cfg, form)
     r = run_oracle(oracle_lib, s, gs, cfg, cull=True)
     assert np.array_equal(bits(g['photon']), bits(r['photon']))
-    sel = (bits(r['photon'])[:, 3, 3] & cfg.hitmask) != 0
+    sel = (bits(r['photon'])[:, 3, 3] & cfg.hitmask) == cfg.hitmask
     assert np.array_equal(bits(g['hit']), bits(r['photon'][sel]))
     assert g['stats']['bounce_steps'] == int(r['bounce'].sum())
 
@@ -275,3 +275,20 @@ def test_trace_bitexact(engine_mod, oracle_lib, name):
     r = o.trace(rays)
     assert np.array_equal(bits(g), bits(r))
     assert (bits(g)[:, 1, 3] & 0xffff != 0xffff).mean() > 0.5
+
+
+def test_multibit_hitmask_requires_all_bits(engine_mod, oracle_lib):
+    """Hit selection requires every bit of the hit mask (sphoton_selector,
+    sysrap/sphoton.h:1219-1225, used by QEvt::gatherHit, qudarap/QEvt.cc:889): with
+    SD|DR only detected photons that were also diffusely reflected are hits."""
+    s = scene('csi_calorimeter')
+    F = abi.FLAGS
+    m = F['SURFACE_DETECT'] | F['SURFACE_DREFLECT']
+    cfg = abi.default_config(**s.config, hitmask=m)
+    gs = s.gensteps(scale_photons=SCENES['csi_calorimeter'][1])
+    g = run_engine(engine_mod, s, gs, cfg)
+    r = run_oracle(oracle_lib, s, gs, cfg, cull=True)
+    u = bits(r['photon'])[:, 3, 3]
+    sel = (u & m) == m
+    assert 0 < sel.sum() < ((u & F['SURFACE_DETECT']) != 0).sum()
+    assert np.array_equal(bits(g['hit']), bits(r['photon'][sel]))
