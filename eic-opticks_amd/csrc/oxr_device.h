@@ -113,21 +113,31 @@ struct Tables {
     int icdf_width, hd_factor;
 };
 
-OXD float4 boundary_lookup(const Tables& t, float nm, unsigned line, unsigned k) {
+// the wavelength coordinate of a lookup: the two texels and their weights (shared by every
+// row looked up at one wavelength)
+struct TexX { int m0, m1; float a, b; };
+OXD TexX tex_x(const Tables& t, float nm) {
     const int nx = t.num_wl;
     float fx = (nm - t.wl_min) / t.wl_step;
     float x = (fx + 0.5f) / float(nx);
-    unsigned iy = 2u * line + k;
     float xf = x * float(nx) - 0.5f;
     float fl = floorf(xf);
-    float a = xf - fl;
+    TexX X;
+    X.a = xf - fl;
+    X.b = 1.0f - X.a;
     int i0 = int(fl);
-    int m0 = i0 % nx; if (m0 < 0) m0 += nx;
-    int m1 = m0 + 1; if (m1 == nx) m1 = 0;
-    const float4* row = t.bnd + size_t(iy) * nx;
-    float4 v0 = row[m0], v1 = row[m1];
-    float b = 1.0f - a;
+    X.m0 = i0 % nx; if (X.m0 < 0) X.m0 += nx;
+    X.m1 = X.m0 + 1; if (X.m1 == nx) X.m1 = 0;
+    return X;
+}
+OXD float4 boundary_fetch(const Tables& t, const TexX& X, unsigned line, unsigned k) {
+    const float4* row = t.bnd + size_t(2u * line + k) * t.num_wl;
+    float4 v0 = row[X.m0], v1 = row[X.m1];
+    const float a = X.a, b = X.b;
     return make_float4(b * v0.x + a * v1.x, b * v0.y + a * v1.y, b * v0.z + a * v1.z, b * v0.w + a * v1.w);
+}
+OXD float4 boundary_lookup(const Tables& t, float nm, unsigned line, unsigned k) {
+    return boundary_fetch(t, tex_x(t, nm), line, k);
 }
 
 OXD float icdf_lookup(const Tables& t, float x, int row) {

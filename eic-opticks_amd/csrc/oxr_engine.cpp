@@ -536,11 +536,15 @@ int oxr_event_set_precooked(oxr_engine* eh, const float* u, int64_t n_photon, in
 }
 
 static int check_sources(Engine& e, int64_t n, uint64_t slot_off) {
-    // INPUT_PHOTON gensteps read input[slot_offset + idx]; precooked rows read pre[idx]
-    bool any_input = false;
-    for (int64_t g = 0; g < e.num_gs; g++) if (fbits(e.h_gs[size_t(24 * g)]) == 19u) any_input = true;
-    if (any_input && (!e.have_input || int64_t(slot_off) + n > e.input_n))
-        return fail(OXR_ESTATE, "INPUT_PHOTON gensteps need oxr_event_set_input_photons covering every photon");
+    // photons of INPUT_PHOTON gensteps read input[slot_offset + idx] (qsim.h:2537 reads the
+    // input array at the absolute photon index); precooked rows read pre[idx]
+    int64_t need = 0, off = 0;
+    for (int64_t g = 0; g < e.num_gs; g++) {
+        off += int64_t(fbits(e.h_gs[size_t(24 * g + 3)]));
+        if (fbits(e.h_gs[size_t(24 * g)]) == 19u) need = int64_t(slot_off) + off;
+    }
+    if (need > 0 && (!e.have_input || need > e.input_n))
+        return fail(OXR_ESTATE, "INPUT_PHOTON gensteps need oxr_event_set_input_photons covering their photon indices");
     if (e.have_pre && e.pre_rows < n)
         return fail(OXR_ESTATE, "precooked stream has fewer rows than photons");
     return OXR_OK;

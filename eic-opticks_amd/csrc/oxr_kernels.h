@@ -14,9 +14,9 @@ struct Seg {
     long long start, count;
     unsigned long long photon0;   // photon index within the event of the first photon
     const float4* gs;
-    const int* seed;              // genstep index per photon of the whole event
+    const long long* off;         // the event's genstep photon offsets (num_gs + 1)
+    int num_gs;
     int event_index;
-    int pad;
 };
 
 // Everything one simulate launch reads or writes (the role of CSGOptiX/Params.h:29-122

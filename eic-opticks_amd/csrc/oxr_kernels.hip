@@ -10,9 +10,9 @@
 
 // Minimum waves per SIMD the simulate kernels are compiled for (VGPR budget 512/N per
 // lane). The latency-bound traversal wants occupancy more than registers: measured on
-// MI355X, 3 for the persistent form and 4 for the plain form are fastest.
+// MI355X (crystal-calorimeter event), 4 waves beat 3 and 5 for both forms.
 #ifndef OXR_PERSIST_WAVES
-#define OXR_PERSIST_WAVES 3
+#define OXR_PERSIST_WAVES 4
 #endif
 #ifndef OXR_PLAIN_WAVES
 #define OXR_PLAIN_WAVES 4
@@ -21,15 +21,20 @@
 namespace oxd {
 
 // ------------------------------------------------------------------ seeding
-__global__ void k_fill_seed(const long long* __restrict__ offsets, int num_gs, int* __restrict__ seed, long long n) {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int lo = 0, hi = num_gs - 1;             // find g with offsets[g] <= i < offsets[g+1]
+// genstep g with offsets[g] <= i < offsets[g+1] (offsets non-decreasing, i < offsets[num_gs])
+OXD int genstep_of(const long long* __restrict__ offsets, int num_gs, long long i) {
+    int lo = 0, hi = num_gs - 1;
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (offsets[mid] <= i) lo = mid; else hi = mid - 1;
     }
-    seed[i] = lo;
+    return lo;
+}
+
+__global__ void k_fill_seed(const long long* __restrict__ offsets, int num_gs, int* __restrict__ seed, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    seed[i] = genstep_of(offsets, num_gs, i);
 }
 
 // ------------------------------------------------------------------ per-photon pieces
@@ -76,7 +81,7 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
         if (li >= sg.count) return false;
         pidx = sg.photon0 + (unsigned long long)li;
         gsb = sg.gs;
-        gsi = sg.seed[pidx];
+        gsi = genstep_of(sg.off, sg.num_gs, (long long)pidx);   // served events carry no seed array
         evt = sg.event_index;
     } else {
         pidx = P.slot_offset + (unsigned long long)idx;

@@ -94,10 +94,11 @@ OXD void fill_state(const Tables& t, St& s, unsigned boundary, float wl, float c
     int m1 = cosTheta > 0.f ? line + 3 : line + 0;
     int m2 = cosTheta > 0.f ? line + 0 : line + 3;
     int su = cosTheta > 0.f ? line + 2 : line + 1;
-    s.m1 = boundary_lookup(t, wl, m1, 0);
-    s.m1g2 = boundary_lookup(t, wl, m1, 1);
-    s.m2 = boundary_lookup(t, wl, m2, 0);
-    s.su = boundary_lookup(t, wl, su, 0);
+    const TexX X = tex_x(t, wl);   // one wavelength coordinate for the four rows
+    s.m1 = boundary_fetch(t, X, m1, 0);
+    s.m1g2 = boundary_fetch(t, X, m1, 1);
+    s.m2 = boundary_fetch(t, X, m2, 0);
+    s.su = boundary_fetch(t, X, su, 0);
     s.opt = t.optical[su];
 }
 

@@ -9,8 +9,14 @@
 //    a segment table; segments start on compaction-chunk boundaries so each request's
 //    hits are a contiguous range of the ordered hit array.
 //  * Overlap: two step slots on two streams. While the device runs step k, the host
-//    copies out and distributes the hits of step k-1; a request completes as soon as
-//    the last step holding its photons has been distributed.
+//    finishes step k-1: each segment's hits are copied device to device into blocks
+//    owned by its request, and a request completes as soon as the last step holding
+//    its photons has been finished. Hits cross PCIe once, in take_hits, straight into
+//    the caller's buffer on the caller's thread.
+//  * No allocation or free on the request path: a request's gensteps and genstep
+//    offsets live in blocks of a server-owned device pool (a hipFree would synchronise
+//    the device and stall the overlapped steps), and served photons find their genstep
+//    by binary search of the offsets instead of a per-photon seed array.
 // Photon random streams are keyed by (event_index, photon index within the event), so a
 // request's result does not depend on what it was batched with.
 #include "oxr_engine.h"
@@ -27,6 +33,32 @@ namespace oxh {
 
 int genstep_offsets(Engine& e, const float* gs, int64_t num_gs, std::vector<long long>& off, int64_t* n);
 
+// Device blocks in power-of-two size classes, kept for reuse until the server ends.
+// Used by the serving thread and by take_hits on the caller's thread.
+struct Pool {
+    std::mutex m;
+    std::multimap<size_t, void*> free_;
+    std::vector<void*> all;
+    ~Pool() { for (void* p : all) (void)hipFree(p); }
+    static size_t cls(size_t bytes) { size_t c = 256; while (c < bytes) c <<= 1; return c; }
+    hipError_t get(size_t bytes, void** p, size_t* cap) {
+        size_t c = cls(bytes);
+        std::lock_guard<std::mutex> lk(m);
+        auto it = free_.find(c);
+        if (it != free_.end()) { *p = it->second; *cap = c; free_.erase(it); return hipSuccess; }
+        hipError_t e = hipMalloc(p, c);
+        if (e == hipSuccess) { all.push_back(*p); *cap = c; }
+        return e;
+    }
+    void put(void* p, size_t cap) {
+        if (!p) return;
+        std::lock_guard<std::mutex> lk(m);
+        free_.emplace(cap, p);
+    }
+};
+
+struct HitBlock { void* p; size_t cap; int64_t n; };
+
 struct Req {
     int64_t id = 0;
     int32_t event_index = 0;
@@ -36,14 +68,16 @@ struct Req {
     double t_submit = 0, t_admit = 0, t_done = 0;
     int status = 0;
     int64_t bytes = 0;
-    DBuf<float4> d_gs;
-    DBuf<int> d_seed;
-    DBuf<long long> d_off;
+    void* d_gs = nullptr;     // pool block: gensteps, then the num_gs+1 offsets
+    size_t d_cap = 0;
     int64_t dispatched = 0;   // photons placed in launched steps
     int inflight = 0;         // launched steps holding its photons, not yet distributed
-    std::vector<float> hits;
+    std::vector<HitBlock> hits;   // device blocks, in photon order
     int64_t nhit = 0;
-    void free_device() { d_gs.release(); d_seed.release(); d_off.release(); }
+    const float4* gs_dev() const { return static_cast<const float4*>(d_gs); }
+    const long long* off_dev() const {
+        return reinterpret_cast<const long long*>(static_cast<const char*>(d_gs) + size_t(num_gs) * 96);
+    }
 };
 
 struct Slot {
@@ -56,8 +90,6 @@ struct Slot {
     std::vector<int64_t> take;
     long long* h_offs = nullptr;    // pinned
     size_t h_offs_n = 0;
-    float* h_hits = nullptr;        // pinned staging
-    size_t h_hits_n = 0;            // floats
     int64_t N = 0;
     bool busy = false;
     uint64_t seqno = 0;
@@ -79,11 +111,14 @@ struct Server {
     Slot slot[2];
     uint64_t next_seq = 0;
     hipStream_t admit_stream = nullptr;
+    Pool pool;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
     double stat_steps = 0, stat_photons = 0, stat_busy_ms = 0, stat_kernel_ms = 0, stat_bounce = 0, stat_hits = 0;
     double clock() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
     void loop();
     int admit_upload(Req& r);
+    void release(Req& r) { pool.put(r.d_gs, r.d_cap); r.d_gs = nullptr; r.d_cap = 0; }
+    void release_hits(Req& r) { for (auto& b : r.hits) pool.put(b.p, b.cap); r.hits.clear(); }
     int launch_step(Slot& s);
     int finalize_step(Slot& s);
     void complete(std::unique_ptr<Req>& r, int status);
@@ -93,24 +128,25 @@ struct Server {
     }
 };
 
+// queue the request's gensteps + offsets for upload on the admission stream (the loop
+// synchronises that stream once per turn, before any step can use them)
 int Server::admit_upload(Req& r) {
-    size_t np = size_t(std::max<int64_t>(r.num_photon, 1));
-    OXR_HIP(r.d_gs.reserve(size_t(std::max<int64_t>(r.num_gs, 1)) * 6));
-    OXR_HIP(r.d_off.reserve(size_t(r.num_gs) + 1));
-    OXR_HIP(r.d_seed.reserve(np));
-    if (r.num_gs > 0) {
-        OXR_HIP(hipMemcpyAsync(r.d_gs.p, r.gs.data(), size_t(r.num_gs) * 96, hipMemcpyHostToDevice, admit_stream));
-        OXR_HIP(hipMemcpyAsync(r.d_off.p, r.off.data(), (size_t(r.num_gs) + 1) * sizeof(long long), hipMemcpyHostToDevice, admit_stream));
-        OXR_HIP(oxl::fill_seed(admit_stream, r.d_off.p, int(r.num_gs), r.d_seed.p, r.num_photon));
-    }
-    OXR_HIP(hipStreamSynchronize(admit_stream));
+    if (r.num_gs <= 0) return OXR_OK;
+    size_t gbytes = size_t(r.num_gs) * 96, obytes = (size_t(r.num_gs) + 1) * sizeof(long long);
+    OXR_HIP(pool.get(gbytes + obytes, &r.d_gs, &r.d_cap));
+    // one host staging copy so a single transfer carries both arrays
+    std::vector<char> h(gbytes + obytes);
+    std::memcpy(h.data(), r.gs.data(), gbytes);
+    std::memcpy(h.data() + gbytes, r.off.data(), obytes);
+    OXR_HIP(hipMemcpyAsync(r.d_gs, h.data(), h.size(), hipMemcpyHostToDevice, admit_stream));
+    OXR_HIP(hipStreamSynchronize(admit_stream));   // h is pageable and local
     return OXR_OK;
 }
 
 void Server::complete(std::unique_ptr<Req>& r, int status) {
     r->status = status;
     r->t_done = clock();
-    r->free_device();
+    release(*r);
     std::lock_guard<std::mutex> lk(mu);
     used -= r->bytes;
     done_ids.push_back(r->id);
@@ -134,7 +170,7 @@ int Server::launch_step(Slot& s) {
         oxd::Seg g{};
         g.start = cursor; g.count = t;
         g.photon0 = uint64_t(r->dispatched);
-        g.gs = r->d_gs.p; g.seed = r->d_seed.p;
+        g.gs = r->gs_dev(); g.off = r->off_dev(); g.num_gs = int(r->num_gs);
         g.event_index = r->event_index;
         s.segs.push_back(g); s.owner.push_back(r); s.take.push_back(t);
         cursor += ((t + CH - 1) / CH) * CH;
@@ -186,30 +222,24 @@ int Server::finalize_step(Slot& s) {
     OXR_HIP(hipStreamSynchronize(s.stream));
     int nb = oxl::compact_blocks(s.N);
     long long nh = s.h_offs[nb];
-    if (nh > 0) {
-        size_t need = size_t(nh) * 16;
-        if (s.h_hits_n < need) {
-            if (s.h_hits) (void)hipHostFree(s.h_hits);
-            s.h_hits = nullptr; s.h_hits_n = 0;
-            OXR_HIP(hipHostMalloc(reinterpret_cast<void**>(&s.h_hits), need * sizeof(float), hipHostMallocDefault));
-            s.h_hits_n = need;
-        }
-        OXR_HIP(hipMemcpyAsync(s.h_hits, s.buf.hit.p, size_t(nh) * 64, hipMemcpyDeviceToHost, s.stream));
-        OXR_HIP(hipStreamSynchronize(s.stream));
-    }
     unsigned long long ctr[8] = {0};
     OXR_HIP(hipMemcpyAsync(ctr, s.buf.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, s.stream));
+    // each segment's hits: device-to-device into a block of its request; the slot's hit
+    // buffer is reused by this slot's next step only after these copies (same stream)
+    for (size_t k = 0; k < s.segs.size(); k++) {
+        long long h0 = s.h_offs[size_t(s.segs[k].start / CH)];
+        long long h1 = k + 1 < s.segs.size() ? s.h_offs[size_t(s.segs[k + 1].start / CH)] : nh;
+        if (h1 <= h0) continue;
+        HitBlock b{nullptr, 0, h1 - h0};
+        OXR_HIP(pool.get(size_t(b.n) * 64, &b.p, &b.cap));
+        s.owner[k]->hits.push_back(b);
+        s.owner[k]->nhit += b.n;
+        OXR_HIP(hipMemcpyAsync(b.p, s.buf.hit.p + 4 * h0, size_t(b.n) * 64, hipMemcpyDeviceToDevice, s.stream));
+    }
     OXR_HIP(hipStreamSynchronize(s.stream));
     if (ctr[4] != 0) return fail(OXR_EINVAL, "traversal stack overflow");
     float kms = 0.f;
     (void)hipEventElapsedTime(&kms, s.e0, s.e1);
-    for (size_t k = 0; k < s.segs.size(); k++) {
-        long long h0 = s.h_offs[size_t(s.segs[k].start / CH)];
-        long long h1 = k + 1 < s.segs.size() ? s.h_offs[size_t(s.segs[k + 1].start / CH)] : nh;
-        Req* r = s.owner[k];
-        if (h1 > h0) r->hits.insert(r->hits.end(), s.h_hits + h0 * 16, s.h_hits + h1 * 16);
-        r->nhit += h1 - h0;
-    }
     auto w1 = std::chrono::steady_clock::now();
     std::lock_guard<std::mutex> lk(mu);
     stat_steps += 1;
@@ -291,7 +321,6 @@ int oxr_server_create(oxr_engine* eh, const oxr_server_config* cfg, oxr_server**
         for (auto& sl : s->slot) {
             sl.buf.release_all(); sl.d_segs.release();
             if (sl.h_offs) (void)hipHostFree(sl.h_offs);
-            if (sl.h_hits) (void)hipHostFree(sl.h_hits);
             if (sl.e0) (void)hipEventDestroy(sl.e0);
             if (sl.e1) (void)hipEventDestroy(sl.e1);
             if (sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -325,13 +354,12 @@ void oxr_server_destroy(oxr_server* sh) {
     }
     s->cv_work.notify_all();
     if (s->th.joinable()) s->th.join();
-    for (auto& r : s->incoming) r->free_device();
-    for (auto& kv : s->finished) kv.second->free_device();
+    for (auto& r : s->incoming) s->release(*r);
+    for (auto& kv : s->finished) { s->release(*kv.second); s->release_hits(*kv.second); }
     for (auto& sl : s->slot) {
         (void)hipStreamSynchronize(sl.stream);
         sl.buf.release_all(); sl.d_segs.release();
         if (sl.h_offs) (void)hipHostFree(sl.h_offs);
-        if (sl.h_hits) (void)hipHostFree(sl.h_hits);
         (void)hipEventDestroy(sl.e0);
         (void)hipEventDestroy(sl.e1);
         (void)hipStreamDestroy(sl.stream);
@@ -348,6 +376,11 @@ double oxr_server_clock(oxr_server* sh) {
 int oxr_server_submit(oxr_server* sh, int64_t id, const float* gs, int64_t num_gs, int32_t event_index) {
     Server* s = reinterpret_cast<Server*>(sh);
     if (!s || num_gs < 0 || (num_gs > 0 && !gs)) return fail(OXR_EINVAL, "bad argument");
+    for (int64_t g = 0; g < num_gs; g++) {
+        uint32_t code;
+        std::memcpy(&code, gs + 24 * g, 4);
+        if (code == 19u) return fail(OXR_EINVAL, "INPUT_PHOTON gensteps are not served (no input photon array per request)");
+    }
     auto r = std::make_unique<Req>();
     r->id = id;
     r->event_index = event_index;
@@ -387,19 +420,28 @@ int oxr_server_wait(oxr_server* sh, oxr_request_result* res, int32_t timeout_ms)
 int oxr_server_take_hits(oxr_server* sh, int64_t id, void* dst, int64_t cap, int64_t* count) {
     Server* s = reinterpret_cast<Server*>(sh);
     if (!s || !count) return fail(OXR_EINVAL, "bad argument");
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->finished.find(id);
-    if (it == s->finished.end()) return fail(OXR_ENOTFOUND, "unknown or not yet completed request");
-    Req& r = *it->second;
-    *count = r.nhit;
-    int64_t m = std::min(r.nhit, std::max<int64_t>(cap, 0));
-    if (m > 0) {
-        if (!dst) return fail(OXR_EINVAL, "null destination");
-        std::memcpy(dst, r.hits.data(), size_t(m) * 64);
+    std::unique_ptr<Req> r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->finished.find(id);
+        if (it == s->finished.end()) return fail(OXR_ENOTFOUND, "unknown or not yet completed request");
+        *count = it->second->nhit;
+        if (cap < it->second->nhit) return fail(OXR_ECAPACITY, "destination smaller than hit count");
+        if (it->second->nhit > 0 && !dst) return fail(OXR_EINVAL, "null destination");
+        r = std::move(it->second);
+        s->finished.erase(it);
     }
-    if (m < r.nhit) return fail(OXR_ECAPACITY, "destination smaller than hit count");
-    s->finished.erase(it);
-    return OXR_OK;
+    // the blocks were filled on a serving stream that was synchronised before the request
+    // completed; this copy runs on the caller's thread
+    char* o = static_cast<char*>(dst);
+    int rc = OXR_OK;
+    for (auto& b : r->hits) {
+        hipError_t e = hipMemcpy(o, b.p, size_t(b.n) * 64, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { rc = hip_fail(e, "take_hits copy"); break; }
+        o += size_t(b.n) * 64;
+    }
+    s->release_hits(*r);
+    return rc;
 }
 
 int oxr_server_stats(oxr_server* sh, double* o) {

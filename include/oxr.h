@@ -297,7 +297,10 @@ int  oxr_server_submit(oxr_server* s, int64_t request_id, const float* gs, int64
                        int32_t event_index);
 /* Wait for the next completed request (any id). timeout_ms < 0 waits forever. */
 int  oxr_server_wait(oxr_server* s, oxr_request_result* res, int32_t timeout_ms);
-/* Copy (and release) the hits of a completed request. */
+/* Copy the hits of a completed request (sphoton, photon order) and release them; the
+ * single device-to-host copy of served hits happens here, on the caller's thread.
+ * capacity below the count returns OXR_ECAPACITY with *count set, copies nothing and
+ * keeps the request for a retry. */
 int  oxr_server_take_hits(oxr_server* s, int64_t request_id, void* dst, int64_t capacity,
                           int64_t* count);
 /* Aggregate: steps run, photons, busy time. out (8) f64. */

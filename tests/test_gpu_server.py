@@ -61,6 +61,8 @@ def test_failed_request_does_not_disturb_others(engine_mod):
         bad = G.cerenkov([10], [[0, 0, 0]], [[0, 0, 1.0]], 1.0, 0.9, 99999, 200.0, 700.0, 1.5, 10.0)
         with pytest.raises(engine_mod.OxrError):
             srv.submit(99, bad, 0)                 # rejected on submit by table validation
+        with pytest.raises(engine_mod.OxrError):
+            srv.submit(98, G.input_photon(10), 0)  # no input-photon array per request
         for k, gs in enumerate(evs):
             srv.submit(k, gs, k)
         srv.submit(50, np.zeros((0, 6, 4), np.float32), 7)   # empty request completes at once
