@@ -96,10 +96,10 @@ struct Builder {
 };
 
 // expand outward so float slab tests stay conservative for the CSG hits inside
-void put_box(const Box& b, float* out6) {
+void put_box(const Box& b, float* out6, double slack) {
     double m = 0.0;
     for (int k = 0; k < 3; k++) m = std::max(m, std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k])));
-    double eps = 1e-5 * m + 1e-3;
+    double eps = 1e-5 * m + 1e-3 + slack;
     for (int k = 0; k < 3; k++) {
         out6[k] = std::nextafter(float(b.lo[k] - eps), -std::numeric_limits<float>::infinity());
         out6[3 + k] = std::nextafter(float(b.hi[k] + eps), std::numeric_limits<float>::infinity());
@@ -108,7 +108,7 @@ void put_box(const Box& b, float* out6) {
 
 } // namespace
 
-Bvh build_bvh(const std::vector<Box>& boxes, int node_base) {
+Bvh build_bvh(const std::vector<Box>& boxes, int node_base, double slack) {
     Bvh out;
     int n = int(boxes.size());
     if (n == 0) { out.root_ref = int32_t(0x80000000u); return out; }
@@ -136,8 +136,8 @@ Bvh build_bvh(const std::vector<Box>& boxes, int node_base) {
     for (size_t k = 0; k < order.size(); k++) {
         const TmpNode& t = B.tmp[order[k]];
         float b0[6], b1[6];
-        put_box(B.tmp[t.left].box, b0);
-        put_box(B.tmp[t.right].box, b1);
+        put_box(B.tmp[t.left].box, b0, slack);
+        put_box(B.tmp[t.right].box, b1, slack);
         NodeOut& o = out.nodes[k];
         o.a[0] = b0[0]; o.a[1] = b0[1]; o.a[2] = b0[2]; o.a[3] = b0[3];
         o.b[0] = b0[4]; o.b[1] = b0[5]; o.b[2] = b1[0]; o.b[3] = b1[1];

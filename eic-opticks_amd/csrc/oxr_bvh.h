@@ -30,7 +30,9 @@ struct Bvh {
 };
 
 // Build over items with the given boxes. node_base is added to every internal ref so
-// several trees can share one concatenated array.
-Bvh build_bvh(const std::vector<Box>& boxes, int node_base);
+// several trees can share one concatenated array. Stored child boxes are widened by a
+// relative and absolute margin plus slack (mm), the bound on the rounding of the
+// device's slab test for ray origins anywhere in the scene.
+Bvh build_bvh(const std::vector<Box>& boxes, int node_base, double slack);
 
 } // namespace oxh

@@ -59,14 +59,19 @@ struct Rng {
     uint32_t overflow;
 };
 
+// Philox4x32-10. Each round's 32x32 products come from one 64-bit multiply
+// (v_mad_u64_u32 gives both halves). The key is the same for every photon of a launch:
+// readfirstlane keeps its schedule in scalar registers.
 OXD void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
                 uint32_t& o0, uint32_t& o1, uint32_t& o2, uint32_t& o3) {
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    k1 = __builtin_amdgcn_readfirstlane(k1);
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0, n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = uint32_t(p1); c2 = n2; c3 = uint32_t(p0);
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
     }
     o0 = c0; o1 = c1; o2 = c2; o3 = c3;
@@ -76,6 +81,16 @@ OXD void rng_init(Rng& r, uint64_t seed, uint64_t offset, uint64_t skip, uint64_
     r.k0 = uint32_t(seed); r.k1 = uint32_t(seed >> 32);
     r.p0 = uint32_t(photon); r.p1 = uint32_t(photon >> 32);
     r.pos = offset + skip * event;
+    r.have = 0;
+    r.b0 = r.b1 = r.b2 = r.b3 = 0u;
+    r.pre = nullptr; r.npre = 0; r.cur = 0; r.overflow = 0;
+}
+
+// continue a stream at uniform number pos (the counter-based stream needs nothing else)
+OXD void rng_resume(Rng& r, uint64_t seed, uint64_t pos, uint64_t photon) {
+    r.k0 = uint32_t(seed); r.k1 = uint32_t(seed >> 32);
+    r.p0 = uint32_t(photon); r.p1 = uint32_t(photon >> 32);
+    r.pos = pos;
     r.have = 0;
     r.b0 = r.b1 = r.b2 = r.b3 = 0u;
     r.pre = nullptr; r.npre = 0; r.cur = 0; r.overflow = 0;

@@ -24,7 +24,7 @@ int hip_fail(hipError_t e, const char* what) {
 void EventBuffers::release_all() {
     gs.release(); gs_off.release(); seed.release(); photon.release(); seq.release(); record.release();
     hitflag.release(); hit.release(); counts.release(); offs.release(); counters.release();
-    input.release(); pre.release(); lite.release(); hitlite.release(); merged.release();
+    input.release(); pre.release(); lite.release(); hitlite.release(); merged.release(); rstate.release();
 }
 
 static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
@@ -60,6 +60,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.record = cfg.max_record > 0 ? b.record.p : nullptr;
     P.max_record = cfg.max_record;
     P.lite = cfg.photonlite ? b.lite.p : nullptr;
+    P.rstate = b.rstate.p;
     P.hitflag = b.hitflag.p;
     P.bounce_out = nullptr; P.overflow_out = nullptr;
     P.counters = b.counters.p;
@@ -87,6 +88,7 @@ int ensure_event_capacity(Engine& e, EventBuffers& b, int64_t n) {
     if (e.cfg.max_record > 0) OXR_HIP(b.record.reserve(4 * N * size_t(e.cfg.max_record)));
     if (e.cfg.photonlite) OXR_HIP(b.lite.reserve(N));
     OXR_HIP(b.hitflag.reserve(N));
+    OXR_HIP(b.rstate.reserve(N));
     OXR_HIP(b.hit.reserve(4 * N));
     int nb = oxl::compact_blocks(int64_t(N));
     OXR_HIP(b.counts.reserve(size_t(nb)));
@@ -333,6 +335,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->solid_num_prim.resize(size_t(g->num_solid));
     e->solid_blas_root.resize(size_t(g->num_solid));
     std::vector<oxh::Box> solid_box(static_cast<size_t>(g->num_solid));
+    std::vector<std::vector<oxh::Box>> prim_boxes(static_cast<size_t>(g->num_solid));
     for (int64_t s = 0; s < g->num_solid; s++) {
         const int32_t* so = g->solid + 12 * s;
         int numPrim = so[4], primOffset = so[5];
@@ -349,13 +352,41 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
             boxes[size_t(p)] = b;
             sb.grow(b);
         }
-        oxh::Bvh bv = oxh::build_bvh(boxes, int(blas_all.size()));
         e->solid_prim_offset[size_t(s)] = primOffset;
         e->solid_num_prim[size_t(s)] = numPrim;
+        solid_box[size_t(s)] = sb;
+        prim_boxes[size_t(s)] = std::move(boxes);
+    }
+    // Scene extent: the largest world coordinate of any instance box (W) and the largest
+    // object-frame distance a world point within W can map to (S). The device slab test
+    // (reciprocal direction, fma with origin*inverse) rounds within a few float ulps of
+    // max(|origin|, |box|, |t*dir|); boxes are widened by 1e-6 of that scale.
+    double W = 0.0, S = 0.0;
+    for (int64_t i = 0; i < g->num_inst; i++) {
+        const float* q = g->inst + 16 * i;
+        int32_t gas;
+        std::memcpy(&gas, q + 7, 4);
+        if (gas < 0 || gas >= g->num_solid) return bail(fail(OXR_EINVAL, "instance " + std::to_string(i) + ": gas_idx outside solids"));
+        const oxh::Box& sb = solid_box[size_t(gas)];
+        for (int c = 0; c < 8; c++) {
+            double p[3] = {(c & 1) ? sb.hi[0] : sb.lo[0], (c & 2) ? sb.hi[1] : sb.lo[1], (c & 4) ? sb.hi[2] : sb.lo[2]};
+            for (int k = 0; k < 3; k++) W = std::max(W, std::fabs(p[0] * q[k] + p[1] * q[4 + k] + p[2] * q[8 + k] + q[12 + k]));
+        }
+    }
+    for (int64_t i = 0; i < g->num_inst; i++) {
+        double Wi[4][3];
+        invert_instance(g->inst + 16 * i, Wi);
+        double lin = 0.0, tr = 0.0;
+        for (int r = 0; r < 3; r++) lin = std::max(lin, std::fabs(Wi[0][r]) + std::fabs(Wi[1][r]) + std::fabs(Wi[2][r]));
+        for (int k = 0; k < 3; k++) tr = std::max(tr, std::fabs(Wi[3][k]));
+        if (std::isfinite(lin) && std::isfinite(tr)) S = std::max(S, 2.0 * lin * W + tr);
+    }
+    const double slack_world = 1e-6 * 2.0 * W, slack_obj = 1e-6 * std::max(S, 2.0 * W);
+    for (int64_t s = 0; s < g->num_solid; s++) {
+        oxh::Bvh bv = oxh::build_bvh(prim_boxes[size_t(s)], int(blas_all.size()), slack_obj);
         e->solid_blas_root[size_t(s)] = bv.root_ref;
         max_blas_depth = std::max(max_blas_depth, bv.depth);
         blas_all.insert(blas_all.end(), bv.nodes.begin(), bv.nodes.end());
-        solid_box[size_t(s)] = sb;
     }
     // ---- instances and TLAS
     std::vector<oxd::InstD> insts(static_cast<size_t>(g->num_inst));
@@ -390,7 +421,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
         }
         inst_box[size_t(i)] = wb;
     }
-    oxh::Bvh tl = oxh::build_bvh(inst_box, 0);
+    oxh::Bvh tl = oxh::build_bvh(inst_box, 0, slack_world);
     e->tlas_root = tl.root_ref;
     e->tlas_nodes = int64_t(tl.nodes.size());
     e->blas_nodes = int64_t(blas_all.size());

@@ -52,6 +52,7 @@ struct EventBuffers {
     DBuf<uint4> lite;          // sphotonlite per photon (config photonlite)
     DBuf<uint4> hitlite;       // selected sphotonlite, formed on demand
     DBuf<uint32_t> merged;     // merge output (sphoton or sphotonlite words)
+    DBuf<ulonglong2> rstate;   // generate -> propagate hand-over, 16 B per photon
     void release_all();
 };
 

@@ -34,6 +34,8 @@ struct Params {
     float4* record;            // records or null
     int max_record;
     uint4* lite;               // sphotonlite out (16 B per photon) or null
+    ulonglong2* rstate;        // generate -> propagate hand-over: {next uniform, photon index};
+                               // x = ~0 marks a padding slot of a batched step
     uint8_t* hitflag;          // 1 if (flagmask & hitmask) == hitmask
     uint32_t* bounce_out;      // optional per-photon loop count
     uint32_t* overflow_out;    // optional precooked-stream wrap count

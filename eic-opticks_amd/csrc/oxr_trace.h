@@ -40,18 +40,22 @@ struct Hit {
     uint32_t ii_id, gpbd;
 };
 
+// Slab-test helpers. These only decide which boxes are entered, never a hit's value:
+// the reciprocal is the hardware approximation (v_rcp_f32, 1 ulp) and the slab
+// distances are fma(bound, inv, -origin*inv). The host widens every stored box by more
+// than the resulting rounding for ray origins anywhere in the scene.
 OXD f3 safe_inv(f3 d) {
     const float e = 1e-20f;
-    return mk(1.f / (fabsf(d.x) > e ? d.x : copysignf(e, d.x)),
-              1.f / (fabsf(d.y) > e ? d.y : copysignf(e, d.y)),
-              1.f / (fabsf(d.z) > e ? d.z : copysignf(e, d.z)));
+    return mk(__builtin_amdgcn_rcpf(fabsf(d.x) > e ? d.x : copysignf(e, d.x)),
+              __builtin_amdgcn_rcpf(fabsf(d.y) > e ? d.y : copysignf(e, d.y)),
+              __builtin_amdgcn_rcpf(fabsf(d.z) > e ? d.z : copysignf(e, d.z)));
 }
-
-// slab test of one child box; boxes are pre-expanded on the host so this is conservative
-OXD bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, f3 o, f3 inv, float tbest, float& tn) {
-    float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
-    float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
-    float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+OXD f3 neg_oi(f3 o, f3 inv) { return mk(-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z)); }
+// slab test of one child box (noi = -origin*inv)
+OXD bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz, f3 noi, f3 inv, float tbest, float& tn) {
+    float tx0 = fmaf(lx, inv.x, noi.x), tx1 = fmaf(hx, inv.x, noi.x);
+    float ty0 = fmaf(ly, inv.y, noi.y), ty1 = fmaf(hy, inv.y, noi.y);
+    float tz0 = fmaf(lz, inv.z, noi.z), tz1 = fmaf(hz, inv.z, noi.z);
     float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fminf(tz0, tz1));
     float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
     tn = n;
@@ -84,7 +88,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
     int sp = 0;
     int cur = g.tlas_root;
     bool at_top = true;
-    f3 o = pos, d = mom, inv = safe_inv(mom);
+    f3 o = pos, d = mom, inv = safe_inv(mom), noi = neg_oi(pos, inv);
     int inst = 0;
     int primOffset = 0;
     for (;;) {
@@ -94,8 +98,8 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             else nd = g.blas[cur];
             if (COUNT) visits++;
             float tn0, tn1;
-            bool h0 = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, o, inv, best, tn0);
-            bool h1 = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, o, inv, best, tn1);
+            bool h0 = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, noi, inv, best, tn0);
+            bool h1 = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, noi, inv, best, tn1);
             if (h0 && h1) {
                 bool first0 = tn0 <= tn1;
                 int nearc = first0 ? nd.ch.x : nd.ch.y;
@@ -114,7 +118,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         }
         if (cur == SENTINEL) {
             at_top = true;
-            o = pos; d = mom; inv = safe_inv(mom);
+            o = pos; d = mom; inv = safe_inv(mom); noi = neg_oi(pos, inv);
             if (sp == 0) break;
             sp--; cur = stk[sp * stride];
             continue;
@@ -128,6 +132,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             if ((in.info.z & 1) == 0) {
                 to_object(in, pos, mom, o, d);
                 inv = safe_inv(d);
+                noi = neg_oi(o, inv);
             }
             if (sp < cap) { stk[sp * stride] = SENTINEL; sp++; } else { err |= 1u; }
             at_top = false;
