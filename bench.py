@@ -46,20 +46,26 @@ def load_tuning(name):
     return None
 
 
-def read_traffic(workload):
-    """HBM bytes per launch of the simulate kernel from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration)."""
+def read_profile(workload):
+    """Figures measured by rocprofv3 for the simulate launch of this workload, from the
+    committed summary profiles/pmc_traffic.txt: 'traffic' = HBM bytes per launch
+    (2.0 x FETCH_SIZE + 1.0 x WRITE_SIZE, copy-kernel calibration), 'valu_busy' and
+    'valu_util' (percent) of the propagate kernel."""
     path = os.path.join(ROOT, 'profiles', 'pmc_traffic.txt')
+    out = {}
     if not os.path.exists(path):
-        return None
+        return out
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == workload:
+        if not parts or parts[0] != workload:
+            continue
+        for kv in parts[1:]:
+            k, _, v = kv.partition('=')
             try:
-                return float(parts[1])
+                out[k] = float(v)
             except ValueError:
-                return None
-    return None
+                pass
+    return out
 
 
 def algorithmic_bytes(stats, n_photon, n_gs, n_hit):
@@ -76,15 +82,18 @@ def algorithmic_bytes(stats, n_photon, n_gs, n_hit):
 
 
 def timed_launches(e, form, steps, warmup):
-    from oxr import abi
+    """Wall time of `steps` synchronous launches; the returned stats carry the mean
+    kernel_ms and generate_ms (HIP events on the engine stream) over those launches."""
     for _ in range(warmup):
         e.launch(form)
-    ks = []
+    ks, gs = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         st = e.launch(form)
         ks.append(st['kernel_ms'])
+        gs.append(st['generate_ms'])
     dt = time.perf_counter() - t0
+    st = dict(st, generate_ms=float(np.mean(gs)))
     return dt, st, float(np.mean(ks))
 
 
@@ -287,10 +296,13 @@ def main():
     cnt = e.launch(counting=True)
     ab = algorithmic_bytes(cnt, n, len(gs), st['num_hit'])
     achieved = ab / (kms / 1e3) / 1e9
-    traffic = read_traffic(name)
+    prof = read_profile(name)
     roof = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel='k_simulate_persist', algorithmic_bytes_per_launch=int(ab),
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=prof.get('traffic'),
+                kernel='k_generate + k_simulate_persist (one simulate launch)',
+                generate_ms=round(st['generate_ms'], 4), propagate_ms=round(kms - st['generate_ms'], 4),
+                valu_busy_pct=prof.get('valu_busy'), valu_lane_util_pct=prof.get('valu_util'),
+                algorithmic_bytes_per_launch=int(ab),
                 node_visits=int(cnt['node_visits']), prim_tests=int(cnt['prim_tests']),
                 csg_node_bytes=int(cnt['csg_node_bytes']), instance_visits=int(cnt['instance_visits']))
     log(f'[bench] roofline: {achieved:.0f} GB/s algorithmic ({ab / 1e9:.2f} GB/launch)')

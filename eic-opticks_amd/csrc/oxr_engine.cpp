@@ -154,6 +154,9 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     // record slots past a photon's last point stay zero, as the reference's zeroed record array
     if (P.record && n > 0) OXR_HIP(hipMemsetAsync(P.record, 0, size_t(n) * size_t(e.cfg.max_record) * 64, s));
     OXR_HIP(hipEventRecord(e.e0, s));
+    bool gen = oxl::needs_generate(P, count);
+    if (gen) OXR_HIP(oxl::generate(s, P));
+    OXR_HIP(hipEventRecord(e.eg, s));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e.tune.blocks_per_cu) * std::max(1, e.num_cu);
         OXR_HIP(oxl::simulate_persist(s, P, block, grid, e.cfg.fp64_roots != 0));
@@ -171,6 +174,8 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     if (ctr[4] != 0) return fail(OXR_EINVAL, "traversal stack overflow in BVH (geometry deeper than the stack)");
     float k_ms = 0.f, c_ms = 0.f;
     OXR_HIP(hipEventElapsedTime(&k_ms, e.e0, e.e1));
+    float g_ms = 0.f;
+    OXR_HIP(hipEventElapsedTime(&g_ms, e.e0, e.eg));
     OXR_HIP(hipEventElapsedTime(&c_ms, e.e1, e.e2));
     auto t1 = std::chrono::steady_clock::now();
     if (st) {
@@ -178,6 +183,7 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
         st->num_hit = nhit;
         st->bounce_steps = ctr[0];
         st->kernel_ms = k_ms;
+        st->generate_ms = g_ms;
         st->compact_ms = c_ms;
         st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         st->node_visits = ctr[2];
@@ -320,6 +326,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
     if ((he = hipEventCreate(&e->e1)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
     if ((he = hipEventCreate(&e->e2)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
+    if ((he = hipEventCreate(&e->eg)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
 
     std::vector<int2> primtab;
     int rc = validate_and_prims(*e, g, ph, primtab);
@@ -474,6 +481,7 @@ void oxr_engine_destroy(oxr_engine* eh) {
     if (e->e0) (void)hipEventDestroy(e->e0);
     if (e->e1) (void)hipEventDestroy(e->e1);
     if (e->e2) (void)hipEventDestroy(e->e2);
+    if (e->eg) (void)hipEventDestroy(e->eg);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -745,6 +753,7 @@ int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_
         total_hit += s.num_hit;
         acc.num_photon += s.num_photon; acc.num_hit += s.num_hit; acc.bounce_steps += s.bounce_steps;
         acc.kernel_ms += s.kernel_ms; acc.compact_ms += s.compact_ms; acc.total_ms += s.total_ms;
+        acc.generate_ms += s.generate_ms;
         slot += uint64_t(n);
         if (num_gs == 0) break;
         g0 = g1;

@@ -90,7 +90,7 @@ struct Engine {
     int num_bnd = 0;
     bool launched = false;
     bool hitlite_ready = false;
-    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, eg = nullptr;
     std::mutex mu;   // serialises C-ABI calls on one engine
 
     oxd::Geom geom() const;

@@ -62,6 +62,10 @@ struct Params {
 namespace oxl {
 hipError_t fill_seed(hipStream_t s, const long long* offsets, int num_gs, int* seed, long long n);
 size_t simulate_smem(const oxd::Params& P, int block);
+// Production launches (no debug outputs, no counting) run generate() first: the
+// propagate kernels then resume the photons it wrote.
+bool needs_generate(const oxd::Params& P, bool count);
+hipError_t generate(hipStream_t s, const oxd::Params& P);
 hipError_t simulate_plain(hipStream_t s, const oxd::Params& P, int block, bool fp64, bool count);
 hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int grid, bool fp64);
 int compact_blocks(long long n);

@@ -197,6 +197,7 @@ int Server::launch_step(Slot& s) {
     P.segs = s.d_segs.p; P.nseg = int(s.segs.size());
     OXR_HIP(hipMemsetAsync(s.buf.counters.p, 0, 8 * sizeof(unsigned long long), s.stream));
     OXR_HIP(hipEventRecord(s.e0, s.stream));
+    if (oxl::needs_generate(P, false)) OXR_HIP(oxl::generate(s.stream, P));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e->tune.blocks_per_cu) * std::max(1, e->num_cu);
         OXR_HIP(oxl::simulate_persist(s.stream, P, block, grid, e->cfg.fp64_roots != 0));

@@ -60,7 +60,7 @@ class LaunchStats(C.Structure):
     _fields_ = [("num_photon", C.c_int64), ("num_hit", C.c_int64), ("bounce_steps", C.c_uint64),
                 ("kernel_ms", C.c_double), ("compact_ms", C.c_double), ("total_ms", C.c_double),
                 ("node_visits", C.c_uint64), ("prim_tests", C.c_uint64),
-                ("csg_node_bytes", C.c_uint64), ("instance_visits", C.c_uint64)]
+                ("csg_node_bytes", C.c_uint64), ("instance_visits", C.c_uint64), ("generate_ms", C.c_double)]
 
 
 class EngineInfo(C.Structure):

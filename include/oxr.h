@@ -152,6 +152,8 @@ typedef struct oxr_launch_stats {
     uint64_t csg_node_bytes;  /* 64 B per CSG node of each tested prim + 64 B per
                                  transformed leaf (when counting is enabled)            */
     uint64_t instance_visits; /* instance records read on entering a BLAS (counting)    */
+    double   generate_ms;     /* the generation pass inside kernel_ms (production
+                                 launches; 0 when debug outputs generate in place)     */
 } oxr_launch_stats;
 
 typedef struct oxr_engine_info {

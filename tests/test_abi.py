@@ -27,7 +27,7 @@ def test_struct_sizes_match_header(engine_mod):
     # sizes implied by include/oxr.h on LP64
     assert C.sizeof(abi.Geometry) == 96
     assert C.sizeof(abi.Config) == 88
-    assert C.sizeof(abi.LaunchStats) == 80
+    assert C.sizeof(abi.LaunchStats) == 88
     assert C.sizeof(abi.Tuning) == 16
     assert C.sizeof(abi.RequestResult) == 64
 
