@@ -39,7 +39,7 @@ enum : uint32_t {
     BOUNDARY_REFLECT = 1u << 10, BOUNDARY_TRANSMIT = 1u << 11
 };
 enum { BREAK = 1, CONTINUE = 2, BOUNDARY = 3 };
-enum { CSG_ZERO = 0, CSG_NODE = 11, CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103,
+enum { CSG_ZERO = 0, CSG_NODE = 11, CSG_CONTIGUOUS = 11, CSG_DISCONTIGUOUS = 12, CSG_OVERLAP = 13, CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103,
        CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110, CSG_CONVEXPOLYHEDRON = 112,
        CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
 
@@ -575,9 +575,151 @@ OXD int classify(const Is& ise, f3 dir, float tmin) {
     return fabsf(ise.w) > tmin ? ((ise.x * dir.x + ise.y * dir.y + ise.z * dir.z) < 0.f ? 0 : 1) : 2;
 }
 
+// ------------------------------------------------------------------ list nodes
+// Compound nodes whose subNum leaf nodes start subOffset nodes after `root` (the prim's
+// first node): CSG_CONTIGUOUS / CSG_DISCONTIGUOUS / CSG_OVERLAP (CSG/csg_intersect_node.h).
+// `is` is written only for a valid intersect. Geometries without list nodes run kernels
+// built with LIST=false, which contain none of this code.
+constexpr float RT_DEFAULT_MAX = 1.e27f;
+constexpr int CONTIGUOUS_MAX_SUB = 8;   // enter/aux/idx arrays of csg_intersect_node.h:470-472
+
+// intersect_node_discontiguous (csg_intersect_node.h:655-686): closest valid sub intersect
+template <bool FP64>
+OXD bool intersect_node_discontiguous(Is& is, const float4* node, int ni, int root, const float4* plan,
+                                      const float4* itra, float t_min, f3 ro, f3 rd) {
+    const unsigned num_sub = __float_as_uint(node[4 * ni].x), offset_sub = __float_as_uint(node[4 * ni].y);
+    Is closest; closest.x = 0.f; closest.y = 0.f; closest.z = 0.f; closest.w = RT_DEFAULT_MAX;
+    for (unsigned isub = 0; isub < num_sub; isub++) {
+        Is s;
+        if (intersect_leaf<FP64>(s, node, root + int(offset_sub + isub), plan, itra, t_min, ro, rd))
+            if (s.w < closest.w) closest = s;
+    }
+    bool valid = closest.w < RT_DEFAULT_MAX;
+    if (valid) is = closest;
+    return valid;
+}
+
+// intersect_node_contiguous (csg_intersect_node.h:393-624): union of touching or
+// overlapping subs. Zeroth pass: nearest enter and exit count; with no exits the ray
+// starts outside and the nearest enter is the answer. Otherwise collect the enters,
+// order them by t (insertion sort of indices), and from each enter that starts before
+// the farthest exit so far, find its exit; the farthest exit is the answer.
+template <bool FP64>
+OXD bool intersect_node_contiguous(Is& is, const float4* node, int ni, int root, const float4* plan,
+                                   const float4* itra, float t_min, f3 ro, f3 rd) {
+    const int num_sub = int(__float_as_uint(node[4 * ni].x)), offset_sub = int(__float_as_uint(node[4 * ni].y));
+    Is nearest_enter; nearest_enter.x = 0.f; nearest_enter.y = 0.f; nearest_enter.z = 0.f; nearest_enter.w = RT_DEFAULT_MAX;
+    Is farthest_exit; farthest_exit.x = 0.f; farthest_exit.y = 0.f; farthest_exit.z = 0.f; farthest_exit.w = t_min;
+    int exit_count = 0;
+    const float propagate_epsilon = 0.0001f;
+    for (int i = 0; i < num_sub; i++) {
+        Is s;
+        if (intersect_leaf<FP64>(s, node, root + offset_sub + i, plan, itra, t_min, ro, rd)) {
+            int st = classify(s, rd, t_min);
+            if (st == 0) { if (s.w < nearest_enter.w) nearest_enter = s; }
+            else if (st == 1) exit_count += 1;
+        }
+    }
+    if (exit_count == 0) {
+        bool valid = nearest_enter.w > t_min && nearest_enter.w < RT_DEFAULT_MAX;
+        if (valid) is = nearest_enter;
+        return valid;
+    }
+    int enter_count = 0;
+    float enter[CONTIGUOUS_MAX_SUB];
+    int aux[CONTIGUOUS_MAX_SUB], idx[CONTIGUOUS_MAX_SUB];
+    for (int isub = 0; isub < num_sub; isub++) {
+        Is s;
+        if (intersect_leaf<FP64>(s, node, root + offset_sub + isub, plan, itra, t_min, ro, rd)) {
+            int st = classify(s, rd, t_min);
+            if (st == 0) {
+                aux[enter_count] = isub; idx[enter_count] = enter_count; enter[enter_count] = s.w;
+                enter_count += 1;
+            } else if (st == 1) {
+                exit_count += 1;
+                if (s.w > farthest_exit.w) farthest_exit = s;
+            }
+        }
+    }
+    for (int i = 1; i < enter_count; i++) {
+        int key = idx[i];
+        int j = i - 1;
+        while (j >= 0 && enter[idx[j]] > enter[key]) { idx[j + 1] = idx[j]; j = j - 1; }
+        idx[j + 1] = key;
+    }
+    for (int i = 0; i < enter_count; i++) {
+        float tminAdvanced = enter[idx[i]] + propagate_epsilon;
+        int isub = aux[idx[i]];
+        if (tminAdvanced < farthest_exit.w) {
+            Is s;
+            if (intersect_leaf<FP64>(s, node, root + offset_sub + isub, plan, itra, tminAdvanced, ro, rd)) {
+                if (classify(s, rd, tminAdvanced) == 1) {
+                    exit_count += 1;
+                    if (s.w > farthest_exit.w) farthest_exit = s;
+                }
+            }
+        }
+    }
+    bool valid = exit_count > 0 && farthest_exit.w > t_min;
+    if (valid) is = farthest_exit;
+    return valid;
+}
+
+// intersect_node_overlap (csg_intersect_node.h:820-895): intersection of all subs; the
+// farthest enter (or, from inside, the nearest exit) when every sub takes part
+template <bool FP64>
+OXD bool intersect_node_overlap(Is& is, const float4* node, int ni, int root, const float4* plan,
+                                const float4* itra, float t_min, f3 ro, f3 rd) {
+    const unsigned num_sub = __float_as_uint(node[4 * ni].x), offset_sub = __float_as_uint(node[4 * ni].y);
+    Is farthest_enter; farthest_enter.x = 0.f; farthest_enter.y = 0.f; farthest_enter.z = 0.f; farthest_enter.w = t_min;
+    Is nearest_exit; nearest_exit.x = 0.f; nearest_exit.y = 0.f; nearest_exit.z = 0.f; nearest_exit.w = RT_DEFAULT_MAX;
+    const float propagate_epsilon = 0.0001f;
+    unsigned enter_count = 0, exit_count = 0;
+    for (unsigned isub = 0; isub < num_sub; isub++) {
+        const int sn = root + int(offset_sub + isub);
+        Is s;
+        if (intersect_leaf<FP64>(s, node, sn, plan, itra, t_min, ro, rd)) {
+            int st = classify(s, rd, t_min);
+            if (st == 0) {
+                enter_count += 1;
+                if (s.w > farthest_enter.w) farthest_enter = s;
+                float tminAdvanced = s.w + propagate_epsilon;
+                Is s2;
+                if (intersect_leaf<FP64>(s2, node, sn, plan, itra, tminAdvanced, ro, rd)) {
+                    if (classify(s2, rd, tminAdvanced) == 1) {
+                        exit_count += 1;
+                        if (s2.w < nearest_exit.w) nearest_exit = s2;
+                    }
+                }
+            } else if (st == 1) {
+                exit_count += 1;
+                if (s.w < nearest_exit.w) nearest_exit = s;
+            }
+        }
+    }
+    bool overlap_all = farthest_enter.w < nearest_exit.w && (enter_count > exit_count ? enter_count : exit_count) == num_sub;
+    if (overlap_all) {
+        if (farthest_enter.w > t_min && farthest_enter.w < RT_DEFAULT_MAX) { is = farthest_enter; return true; }
+        if (nearest_exit.w > t_min && nearest_exit.w < RT_DEFAULT_MAX) { is = nearest_exit; return true; }
+    }
+    return false;
+}
+
+// intersect_node (csg_intersect_node.h:914-938): list node or leaf
+template <bool FP64, bool LIST>
+OXD bool intersect_node(Is& is, const float4* node, int ni, int root, const float4* plan, const float4* itra,
+                        float t_min, f3 ro, f3 rd) {
+    if (!LIST) return intersect_leaf<FP64>(is, node, ni, plan, itra, t_min, ro, rd);
+    const unsigned tc = __float_as_uint(node[4 * ni + 3].z);
+    if (tc == CSG_CONTIGUOUS) { is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f; return intersect_node_contiguous<FP64>(is, node, ni, root, plan, itra, t_min, ro, rd); }
+    if (tc == CSG_DISCONTIGUOUS) { is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f; return intersect_node_discontiguous<FP64>(is, node, ni, root, plan, itra, t_min, ro, rd); }
+    if (tc == CSG_OVERLAP) { is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f; return intersect_node_overlap<FP64>(is, node, ni, root, plan, itra, t_min, ro, rd); }
+    return intersect_leaf<FP64>(is, node, ni, plan, itra, t_min, ro, rd);
+}
+
 // intersect_tree (CSG/csg_intersect_tree.h:277-672): postorder evaluation with a tranche
 // stack of subtree slices and a stack of partial intersects; side recorded in sign(t).
-template <bool FP64>
+template <bool FP64, bool LIST>
 OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     const unsigned numNode = __float_as_uint(node[4 * root].x);
@@ -606,7 +748,7 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
             }
             if (tc >= CSG_NODE) {
                 Is ni_is;
-                intersect_leaf<FP64>(ni_is, node, ni, plan, itra, tmin, ro, rd);
+                intersect_node<FP64, LIST>(ni_is, node, ni, root, plan, itra, tmin, ro, rd);
                 ni_is.w = copysignf(ni_is.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
                 if (sp >= 14) { ierr = 1; break; }
                 st[++sp] = ni_is;
@@ -664,12 +806,13 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
 }
 
 // intersect_prim (CSG/csg_intersect_tree.h:684-719)
-template <bool FP64>
+template <bool FP64, bool LIST>
 OXD bool intersect_prim(Is& is, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     unsigned tc = __float_as_uint(node[4 * root + 3].z);
     if (tc >= CSG_LEAF) return intersect_leaf<FP64>(is, node, root, plan, itra, t_min, ro, rd);
-    if (tc < CSG_NODE) return intersect_tree<FP64>(is, node, root, plan, itra, t_min, ro, rd);
+    if (tc < CSG_NODE) return intersect_tree<FP64, LIST>(is, node, root, plan, itra, t_min, ro, rd);
+    if (LIST) return intersect_node<FP64, LIST>(is, node, root, root, plan, itra, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     return false;
 }

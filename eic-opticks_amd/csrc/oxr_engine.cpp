@@ -159,9 +159,9 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     OXR_HIP(hipEventRecord(e.eg, s));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e.tune.blocks_per_cu) * std::max(1, e.num_cu);
-        OXR_HIP(oxl::simulate_persist(s, P, block, grid, e.cfg.fp64_roots != 0));
+        OXR_HIP(oxl::simulate_persist(s, P, block, grid, e.cfg.fp64_roots != 0, e.has_list));
     } else {
-        OXR_HIP(oxl::simulate_plain(s, P, block, e.cfg.fp64_roots != 0, count));
+        OXR_HIP(oxl::simulate_plain(s, P, block, e.cfg.fp64_roots != 0, count, e.has_list));
     }
     OXR_HIP(hipEventRecord(e.e1, s));
     OXR_HIP(oxl::compact(s, b.hitflag.p, b.photon.p, n, b.counts.p, b.offs.p, b.hit.p, n));
@@ -221,46 +221,79 @@ static bool supported_leaf(uint32_t tc) {
     return tc == 101 || tc == 103 || tc == 105 || tc == 108 || tc == 110 || tc == 112 || tc == 121 || tc == 125;
 }
 
+static bool is_list(uint32_t tc) { return tc == 11 || tc == 12 || tc == 13; }
+static bool is_op(uint32_t tc) { return tc == 1 || tc == 2 || tc == 3; }
+
+// Checks every prim against what the kernels assume, so no launch can index outside the
+// arrays: node ranges, tree shape (root subNum = 2^k-1 tree nodes, operators above the
+// leaf level), list nodes (CSG_CONTIGUOUS/DISCONTIGUOUS/OVERLAP: subs inside the prim and
+// all leaves, at most 8 for contiguous), leaf typecodes, transform and plane ranges,
+// boundary indices. Sets e.has_list when any prim holds a list node.
 static int validate_and_prims(Engine& e, const oxr_geometry* g, const oxr_physics* ph, std::vector<int2>& primtab) {
     const uint32_t* nu = reinterpret_cast<const uint32_t*>(g->node);
     auto node_u = [&](int64_t i, int k) { return nu[16 * i + k]; };
     primtab.resize(size_t(g->num_prim));
+    e.has_list = false;
     for (int64_t p = 0; p < g->num_prim; p++) {
         const int32_t* pi = reinterpret_cast<const int32_t*>(g->prim + 16 * p);
         const uint32_t* pu = reinterpret_cast<const uint32_t*>(g->prim + 16 * p);
         int32_t numNode = pi[0], nodeOffset = pi[1];
+        std::string tag = "prim " + std::to_string(p) + ": ";
         if (numNode < 1 || nodeOffset < 0 || int64_t(nodeOffset) + numNode > g->num_node)
-            return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": node range outside node array");
-        if (numNode > 255 || ((numNode + 1) & numNode) != 0)
-            return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": numNode must be 2^k-1 and <= 255");
-        int height = 0;
-        while ((1 << (height + 1)) - 1 < numNode) height++;
-        for (int i = 1; i <= numNode; i++) {
-            int64_t ni = nodeOffset + i - 1;
+            return fail(OXR_EINVAL, tag + "node range outside node array");
+        auto check_leaf = [&](int64_t ni) -> int {
             uint32_t tc = node_u(ni, 14);
-            int depth = 31 - __builtin_clz(unsigned(i));
-            bool op = tc == 1 || tc == 2 || tc == 3;
-            if (numNode == 1 && !supported_leaf(tc))
-                return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": unsupported leaf typecode " + std::to_string(tc));
-            if (numNode > 1) {
-                if (i == 1 && !op) return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": tree root must be an operator");
-                if (op && depth == height) return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": operator at leaf level");
-                if (!op && tc != 0 && !supported_leaf(tc))
-                    return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": unsupported node typecode " + std::to_string(tc));
+            if (!supported_leaf(tc)) return fail(OXR_EINVAL, tag + "unsupported leaf typecode " + std::to_string(tc));
+            uint32_t gt = node_u(ni, 15) & 0x7fffffffu;
+            if (int64_t(gt) > g->num_itra) return fail(OXR_EINVAL, tag + "node transform index outside itra");
+            if (tc == 112) {
+                uint32_t pidx = node_u(ni, 0), pnum = node_u(ni, 1);
+                if (int64_t(pidx) + int64_t(pnum) > g->num_plan) return fail(OXR_EINVAL, tag + "convexpolyhedron planes outside plan array");
             }
-            if (tc != 0 && !op) {
-                uint32_t gt = node_u(ni, 15) & 0x7fffffffu;
-                if (int64_t(gt) > g->num_itra) return fail(OXR_EINVAL, "node transform index outside itra");
-                if (tc == 112) {
-                    uint32_t pidx = node_u(ni, 0), pnum = node_u(ni, 1);
-                    if (int64_t(pidx) + int64_t(pnum) > g->num_plan) return fail(OXR_EINVAL, "convexpolyhedron planes outside plan array");
+            return OXR_OK;
+        };
+        auto check_list = [&](int64_t ni) -> int {
+            uint32_t tc = node_u(ni, 14), num_sub = node_u(ni, 0), off_sub = node_u(ni, 1);
+            if (num_sub < 1 || int64_t(off_sub) + int64_t(num_sub) > numNode)
+                return fail(OXR_EINVAL, tag + "list node subs outside the prim's nodes");
+            if (tc == 11 && num_sub > 8) return fail(OXR_EINVAL, tag + "CSG_CONTIGUOUS with more than 8 subs");
+            for (uint32_t k = 0; k < num_sub; k++) {
+                int rc = check_leaf(nodeOffset + int64_t(off_sub) + k);
+                if (rc) return rc;
+            }
+            e.has_list = true;
+            return OXR_OK;
+        };
+        uint32_t rtc = node_u(nodeOffset, 14);
+        int rc = OXR_OK;
+        if (rtc >= 101) {
+            rc = check_leaf(nodeOffset);
+        } else if (is_list(rtc)) {
+            rc = check_list(nodeOffset);
+        } else if (is_op(rtc)) {
+            uint32_t T = node_u(nodeOffset, 0);
+            if (T < 3 || T > 255 || ((T + 1) & T) != 0 || int64_t(T) > numNode)
+                return fail(OXR_EINVAL, tag + "tree root subNum must be 2^k-1 <= 255 and within numNode");
+            int height = 0;
+            while ((1 << (height + 1)) - 1 < int(T)) height++;
+            for (int i = 1; i <= int(T) && rc == OXR_OK; i++) {
+                int64_t ni = nodeOffset + i - 1;
+                uint32_t tc = node_u(ni, 14);
+                int depth = 31 - __builtin_clz(unsigned(i));
+                if (is_op(tc)) {
+                    if (depth == height) rc = fail(OXR_EINVAL, tag + "operator at leaf level");
+                } else if (is_list(tc)) {
+                    rc = check_list(ni);
+                } else if (tc != 0) {
+                    rc = check_leaf(ni);
                 }
             }
+        } else {
+            return fail(OXR_EINVAL, tag + "unsupported root typecode " + std::to_string(rtc));
         }
-        if (numNode > 1 && int(node_u(nodeOffset, 0)) != numNode)
-            return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": root subNum differs from numNode");
+        if (rc) return rc;
         uint32_t bnd = node_u(nodeOffset, 6);
-        if (int64_t(bnd) >= int64_t(ph->num_bnd)) return fail(OXR_EINVAL, "prim " + std::to_string(p) + ": boundary outside bnd table");
+        if (int64_t(bnd) >= int64_t(ph->num_bnd)) return fail(OXR_EINVAL, tag + "boundary outside bnd table");
         uint32_t gp = pu[15];
         primtab[size_t(p)] = make_int2(nodeOffset, int(((gp & 0xffffu) << 16) | (bnd & 0xffffu)));
     }
@@ -776,7 +809,7 @@ int oxr_trace_rays(oxr_engine* eh, const float* rays, int64_t n, float* prd) {
     if ((x = hipMemcpy(dr.p, rays, size_t(n) * 32, hipMemcpyHostToDevice)) != hipSuccess) { cleanup(); return hip_fail(x, "copy rays"); }
     EventBuffers dummy;
     oxd::Params P = e->params(dummy, 0, 0, 0, e->ntop_for(256));
-    if ((x = oxl::trace_rays(e->stream, P, dr.p, n, dp.p, e->cfg.fp64_roots != 0)) != hipSuccess) { cleanup(); return hip_fail(x, "trace launch"); }
+    if ((x = oxl::trace_rays(e->stream, P, dr.p, n, dp.p, e->cfg.fp64_roots != 0, e->has_list)) != hipSuccess) { cleanup(); return hip_fail(x, "trace launch"); }
     if ((x = hipStreamSynchronize(e->stream)) != hipSuccess) { cleanup(); return hip_fail(x, "trace sync"); }
     x = hipMemcpy(prd, dp.p, size_t(n) * 32, hipMemcpyDeviceToHost);
     cleanup();
@@ -800,7 +833,7 @@ int oxr_intersect_prim(oxr_engine* eh, int32_t prim_index, const float* rays, in
         cleanup(); return hip_fail(x, "hipMalloc isect");
     }
     if ((x = hipMemcpy(dr.p, rays, size_t(n) * 32, hipMemcpyHostToDevice)) != hipSuccess) { cleanup(); return hip_fail(x, "copy rays"); }
-    if ((x = oxl::intersect_prim(e->stream, e->geom(), pr.x, dr.p, n, di.p, dv.p, e->cfg.fp64_roots != 0)) != hipSuccess) { cleanup(); return hip_fail(x, "isect launch"); }
+    if ((x = oxl::intersect_prim(e->stream, e->geom(), pr.x, dr.p, n, di.p, dv.p, e->cfg.fp64_roots != 0, e->has_list)) != hipSuccess) { cleanup(); return hip_fail(x, "isect launch"); }
     if ((x = hipStreamSynchronize(e->stream)) != hipSuccess) { cleanup(); return hip_fail(x, "isect sync"); }
     if ((x = hipMemcpy(isect, di.p, size_t(n) * 16, hipMemcpyDeviceToHost)) != hipSuccess) { cleanup(); return hip_fail(x, "copy isect"); }
     x = hipMemcpy(valid, dv.p, size_t(n) * 4, hipMemcpyDeviceToHost);

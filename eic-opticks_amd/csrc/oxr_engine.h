@@ -88,6 +88,7 @@ struct Engine {
     int32_t npre = 0;
     int64_t pre_rows = 0;
     int num_bnd = 0;
+    bool has_list = false;   // some prim holds a list node: launches use the LIST kernels
     bool launched = false;
     bool hitlite_ready = false;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, eg = nullptr;

@@ -66,8 +66,8 @@ size_t simulate_smem(const oxd::Params& P, int block);
 // propagate kernels then resume the photons it wrote.
 bool needs_generate(const oxd::Params& P, bool count);
 hipError_t generate(hipStream_t s, const oxd::Params& P);
-hipError_t simulate_plain(hipStream_t s, const oxd::Params& P, int block, bool fp64, bool count);
-hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int grid, bool fp64);
+hipError_t simulate_plain(hipStream_t s, const oxd::Params& P, int block, bool fp64, bool count, bool list);
+hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int grid, bool fp64, bool list);
 int compact_blocks(long long n);
 hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
                    long long* offs, float4* hit, long long hit_cap);
@@ -77,7 +77,8 @@ hipError_t scatter_lite(hipStream_t s, const uint8_t* flag, const uint4* lite, l
 // merge of sphoton (lite=false) or sphotonlite items by (identity, time bucket); see oxr_merge.hip
 hipError_t merge(hipStream_t s, bool lite, const void* in, long long n, unsigned select_mask, bool select,
                  float time_window, void* out, long long* num_out);
-hipError_t trace_rays(hipStream_t s, const oxd::Params& P, const float4* rays, long long n, float4* prd, bool fp64);
+hipError_t trace_rays(hipStream_t s, const oxd::Params& P, const float4* rays, long long n, float4* prd, bool fp64,
+                      bool list);
 hipError_t intersect_prim(hipStream_t s, const oxd::Geom& g, int root, const float4* rays, long long n,
-                          float4* isect, int* valid, bool fp64);
+                          float4* isect, int* valid, bool fp64, bool list);
 } // namespace oxl

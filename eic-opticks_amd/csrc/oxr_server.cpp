@@ -200,9 +200,9 @@ int Server::launch_step(Slot& s) {
     if (oxl::needs_generate(P, false)) OXR_HIP(oxl::generate(s.stream, P));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e->tune.blocks_per_cu) * std::max(1, e->num_cu);
-        OXR_HIP(oxl::simulate_persist(s.stream, P, block, grid, e->cfg.fp64_roots != 0));
+        OXR_HIP(oxl::simulate_persist(s.stream, P, block, grid, e->cfg.fp64_roots != 0, e->has_list));
     } else {
-        OXR_HIP(oxl::simulate_plain(s.stream, P, block, e->cfg.fp64_roots != 0, false));
+        OXR_HIP(oxl::simulate_plain(s.stream, P, block, e->cfg.fp64_roots != 0, false, e->has_list));
     }
     OXR_HIP(hipEventRecord(s.e1, s.stream));
     OXR_HIP(oxl::compact(s.stream, s.buf.hitflag.p, s.buf.photon.p, s.N, s.buf.counts.p, s.buf.offs.p, s.buf.hit.p, s.N));

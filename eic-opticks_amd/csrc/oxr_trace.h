@@ -77,7 +77,7 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 // During traversal only (t, instance, local prim, object-frame normal) of the best hit
 // stay live; its world normal, local position and identity words are formed afterwards
 // from the instance record, with the same arithmetic as at instance entry.
-template <bool FP64, bool COUNT>
+template <bool FP64, bool COUNT, bool LIST>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
@@ -152,7 +152,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             }
         }
         Is is;
-        if (intersect_prim<FP64>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
+        if (intersect_prim<FP64, LIST>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
             float t = is.w;
             bool closer = t < best || (found && t == best && (inst < best_inst || (inst == best_inst && item < best_item)));
             if (closer) {

@@ -98,8 +98,9 @@ class Node:
     """A CSG node before flattening: leaf params or an operator with children."""
 
     def __init__(self, typecode, param=(0, 0, 0, 0, 0, 0), aabb=None, left=None,
-                 right=None, complement=False, transform=None, planes=None):
+                 right=None, complement=False, transform=None, planes=None, subs=None):
         self.typecode = typecode
+        self.subs = subs                      # leaf Nodes of a list node (CSG_CONTIGUOUS..OVERLAP)
         self.param = list(param)
         self.aabb = None if aabb is None else np.asarray(aabb, dtype=np.float64)
         self.left, self.right = left, right
@@ -119,6 +120,8 @@ class Node:
 
     def moved(self, m):
         """Return self with transform m applied after any existing one."""
+        if self.subs is not None:
+            return Node(self.typecode, subs=[n.moved(m) for n in self.subs])
         if self.typecode < CSG_SPHERE and self.typecode != CSG_ZERO:
             return Node(self.typecode, left=self.left.moved(m), right=self.right.moved(m),
                         complement=self.complement)
@@ -173,8 +176,32 @@ def phicut(phi0_pi, phi1_pi):
 
 
 def complement(n):
+    if n.subs is not None:
+        raise ValueError('list nodes cannot be complemented')
     n2 = Node(n.typecode, n.param, n.aabb, n.left, n.right, not n.complement, n.transform, n.planes)
     return n2
+
+
+def _list(typecode, leaves):
+    leaves = list(leaves)
+    if not leaves or any(l.left is not None or l.subs is not None for l in leaves):
+        raise ValueError('a list node takes one or more leaf nodes')
+    return Node(typecode, subs=leaves)
+
+
+def contiguous(*leaves):
+    """Union of touching or overlapping leaves as one list node (CSG_CONTIGUOUS, <= 8 leaves)."""
+    return _list(CSG_CONTIGUOUS, leaves)
+
+
+def discontiguous(*leaves):
+    """Union of leaves that do not touch (CSG_DISCONTIGUOUS)."""
+    return _list(CSG_DISCONTIGUOUS, leaves)
+
+
+def overlap(*leaves):
+    """Intersection of leaves (CSG_OVERLAP)."""
+    return _list(CSG_OVERLAP, leaves)
 
 
 # ---------------------------------------------------------------------------
@@ -199,8 +226,19 @@ def _positivize(n: Node, neg=False) -> Node:
     return Node(op, left=l, right=r)
 
 
+def _leaf_aabb(n: Node):
+    return n.aabb if n.transform is None else transform_aabb(n.aabb, n.transform)
+
+
 def _aabb_of(n: Node):
     """Conservative world AABB of a positivised tree; None means unbounded."""
+    if n.subs is not None:
+        bs = [_leaf_aabb(x) for x in n.subs if not x.complement]
+        if n.typecode == CSG_OVERLAP:
+            return np.concatenate([np.max([b[:3] for b in bs], axis=0), np.min([b[3:] for b in bs], axis=0)])
+        if len(bs) < len(n.subs):
+            return None
+        return np.concatenate([np.min([b[:3] for b in bs], axis=0), np.max([b[3:] for b in bs], axis=0)])
     if n.left is None:
         if n.complement:
             return None
@@ -300,12 +338,27 @@ class Foundry:
                 place(n.right, 2 * i + 1)
 
         place(t, 1)
-        node_offset = len(self.nodes)
+        # list nodes keep their leaves after the tree nodes: subOffset counts from the
+        # prim's first node (CSGNode subNum/subOffset, CSG/CSGNode.h:187-211)
+        extra = []
+        sub_at = {}
         for i, n in enumerate(flat):
+            if n is not None and n.subs is not None:
+                sub_at[i] = num + len(extra)
+                extra.extend(n.subs)
+        node_offset = len(self.nodes)
+        for i, n in enumerate(flat + extra):
             q = np.zeros(16, dtype=np.float32)
             u = q.view(np.uint32)
             if n is None:
                 u[14] = CSG_ZERO
+            elif n.subs is not None:
+                u[14] = n.typecode
+                u[0] = len(n.subs)
+                u[1] = sub_at[i]
+                aabb = _aabb_of(n)
+                q[8:12] = aabb[0:4]
+                q[12:14] = aabb[4:6]
             elif n.left is not None:
                 u[14] = n.typecode
                 if i == 0:
@@ -337,7 +390,7 @@ class Foundry:
         p = np.zeros(16, dtype=np.float32)
         pi = p.view(np.int32)
         pu = p.view(np.uint32)
-        pi[0] = num
+        pi[0] = num + len(extra)
         pi[1] = node_offset
         pi[2] = 0
         pi[3] = 0

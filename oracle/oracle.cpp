@@ -596,8 +596,135 @@ inline unsigned postorder_next(unsigned i, unsigned elevation) {
 }
 inline unsigned pack4(unsigned c, unsigned d) { return ((c & 0xffu) << 16) | ((d & 0xffu) << 24); }
 
+// ---------------------------------------------------------------- list nodes
+// subNum leaf nodes starting subOffset nodes after the prim's first node `root`
+// (CSG/csg_intersect_node.h); `is` is written only for a valid intersect.
+constexpr float RT_DEFAULT_MAX = 1.e27f;
+
+// intersect_node_discontiguous (csg_intersect_node.h:655-686)
 template <typename T>
-void intersect_node(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, T tmin, V3<T> ro, V3<T> rd);
+void intersect_node_discontiguous(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, const Node* root, T t_min,
+                                  V3<T> ro, V3<T> rd) {
+    unsigned num_sub = nd->u(0), offset_sub = nd->u(1);
+    Isect<T> closest{0, 0, 0, T(RT_DEFAULT_MAX)};
+    for (unsigned isub = 0; isub < num_sub; isub++) {
+        Isect<T> sub{0, 0, 0, 0};
+        bool v = false;
+        intersect_leaf(c, v, sub, root[offset_sub + isub], t_min, ro, rd);
+        if (v && sub.w < closest.w) closest = sub;
+    }
+    valid = closest.w < T(RT_DEFAULT_MAX);
+    if (valid) is = closest;
+}
+
+// intersect_node_contiguous (csg_intersect_node.h:393-624): zeroth pass for the nearest
+// enter and the exit count; from inside, t-ordered enters extend the farthest exit
+template <typename T>
+void intersect_node_contiguous(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, const Node* root, T t_min,
+                               V3<T> ro, V3<T> rd) {
+    const int num_sub = int(nd->u(0)), offset_sub = int(nd->u(1));
+    Isect<T> nearest_enter{0, 0, 0, T(RT_DEFAULT_MAX)};
+    Isect<T> farthest_exit{0, 0, 0, t_min};
+    int exit_count = 0;
+    const T propagate_epsilon = T(0.0001f);
+    for (int i = 0; i < num_sub; i++) {
+        Isect<T> sub{0, 0, 0, 0};
+        bool v = false;
+        intersect_leaf(c, v, sub, root[offset_sub + i], t_min, ro, rd);
+        if (v) {
+            int st = classify(sub, rd, t_min);
+            if (st == State_Enter) { if (sub.w < nearest_enter.w) nearest_enter = sub; }
+            else if (st == State_Exit) exit_count += 1;
+        }
+    }
+    if (exit_count == 0) {
+        valid = nearest_enter.w > t_min && nearest_enter.w < T(RT_DEFAULT_MAX);
+        if (valid) is = nearest_enter;
+        return;
+    }
+    int enter_count = 0;
+    T enter[8];
+    int aux[8], idx[8];
+    for (int isub = 0; isub < num_sub; isub++) {
+        Isect<T> sub{0, 0, 0, 0};
+        bool v = false;
+        intersect_leaf(c, v, sub, root[offset_sub + isub], t_min, ro, rd);
+        if (v) {
+            int st = classify(sub, rd, t_min);
+            if (st == State_Enter) {
+                aux[enter_count] = isub; idx[enter_count] = enter_count; enter[enter_count] = sub.w;
+                enter_count += 1;
+            } else if (st == State_Exit) {
+                exit_count += 1;
+                if (sub.w > farthest_exit.w) farthest_exit = sub;
+            }
+        }
+    }
+    for (int i = 1; i < enter_count; i++) {          // insertion sort of indices by enter t
+        int key = idx[i];
+        int j = i - 1;
+        while (j >= 0 && enter[idx[j]] > enter[key]) { idx[j + 1] = idx[j]; j = j - 1; }
+        idx[j + 1] = key;
+    }
+    for (int i = 0; i < enter_count; i++) {
+        T tminAdvanced = enter[idx[i]] + propagate_epsilon;
+        int isub = aux[idx[i]];
+        if (tminAdvanced < farthest_exit.w) {
+            Isect<T> sub{0, 0, 0, 0};
+            bool v = false;
+            intersect_leaf(c, v, sub, root[offset_sub + isub], tminAdvanced, ro, rd);
+            if (v && classify(sub, rd, tminAdvanced) == State_Exit) {
+                exit_count += 1;
+                if (sub.w > farthest_exit.w) farthest_exit = sub;
+            }
+        }
+    }
+    valid = exit_count > 0 && farthest_exit.w > t_min;
+    if (valid) is = farthest_exit;
+}
+
+// intersect_node_overlap (csg_intersect_node.h:820-895)
+template <typename T>
+void intersect_node_overlap(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, const Node* root, T t_min,
+                            V3<T> ro, V3<T> rd) {
+    unsigned num_sub = nd->u(0), offset_sub = nd->u(1);
+    Isect<T> farthest_enter{0, 0, 0, t_min};
+    Isect<T> nearest_exit{0, 0, 0, T(RT_DEFAULT_MAX)};
+    const T propagate_epsilon = T(0.0001f);
+    unsigned enter_count = 0, exit_count = 0;
+    for (unsigned isub = 0; isub < num_sub; isub++) {
+        const Node& sn = root[offset_sub + isub];
+        Isect<T> sub{0, 0, 0, 0};
+        bool v = false;
+        intersect_leaf(c, v, sub, sn, t_min, ro, rd);
+        if (!v) continue;
+        int st = classify(sub, rd, t_min);
+        if (st == State_Enter) {
+            enter_count += 1;
+            if (sub.w > farthest_enter.w) farthest_enter = sub;
+            T tminAdvanced = sub.w + propagate_epsilon;
+            Isect<T> s2{0, 0, 0, 0};
+            bool v2 = false;
+            intersect_leaf(c, v2, s2, sn, tminAdvanced, ro, rd);
+            if (v2 && classify(s2, rd, tminAdvanced) == State_Exit) {
+                exit_count += 1;
+                if (s2.w < nearest_exit.w) nearest_exit = s2;
+            }
+        } else if (st == State_Exit) {
+            exit_count += 1;
+            if (sub.w < nearest_exit.w) nearest_exit = sub;
+        }
+    }
+    valid = false;
+    bool overlap_all = farthest_enter.w < nearest_exit.w && std::max(enter_count, exit_count) == num_sub;
+    if (overlap_all) {
+        if (farthest_enter.w > t_min && farthest_enter.w < T(RT_DEFAULT_MAX)) { valid = true; is = farthest_enter; }
+        else if (nearest_exit.w > t_min && nearest_exit.w < T(RT_DEFAULT_MAX)) { valid = true; is = nearest_exit; }
+    }
+}
+
+template <typename T>
+void intersect_node(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, const Node* root, T tmin, V3<T> ro, V3<T> rd);
 
 // intersect_tree (CSG/csg_intersect_tree.h:277-672)
 template <typename T>
@@ -630,7 +757,7 @@ void intersect_tree(const Ctx& c, bool& valid, Isect<T>& is, const Node* root, T
             if (node_or_leaf) {
                 Isect<T> ni{0, 0, 0, 0};
                 bool nv = false;
-                intersect_node(c, nv, ni, nd, tmin, ro, rd);
+                intersect_node(c, nv, ni, nd, root, tmin, ro, rd);
                 ni.w = std::copysign(ni.w, nodeIdx % 2 == 0 ? T(-1) : T(1));
                 ierr = spush(ni);
                 if (ierr) break;
@@ -693,9 +820,13 @@ void intersect_tree(const Ctx& c, bool& valid, Isect<T>& is, const Node* root, T
 // intersect_node_discontiguous / overlap / contiguous are list types not used by the
 // benchmark geometry; intersect_node dispatches leaves (csg_intersect_node.h:914-938).
 template <typename T>
-void intersect_node(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, T tmin, V3<T> ro, V3<T> rd) {
+void intersect_node(const Ctx& c, bool& valid, Isect<T>& is, const Node* nd, const Node* root, T tmin, V3<T> ro, V3<T> rd) {
     valid = false;
-    intersect_leaf(c, valid, is, *nd, tmin, ro, rd);
+    unsigned tc = nd->u(14);
+    if (tc == CSG_CONTIGUOUS) intersect_node_contiguous(c, valid, is, nd, root, tmin, ro, rd);
+    else if (tc == CSG_DISCONTIGUOUS) intersect_node_discontiguous(c, valid, is, nd, root, tmin, ro, rd);
+    else if (tc == CSG_OVERLAP) intersect_node_overlap(c, valid, is, nd, root, tmin, ro, rd);
+    else intersect_leaf(c, valid, is, *nd, tmin, ro, rd);
 }
 
 // intersect_prim (CSG/csg_intersect_tree.h:684-719)
@@ -706,6 +837,7 @@ bool intersect_prim(const Ctx& c, Isect<T>& is, const Node* root, T t_min, V3<T>
     is = Isect<T>{0, 0, 0, 0};
     if (typecode >= CSG_LEAF) intersect_leaf(c, valid, is, *root, t_min, ro, rd);
     else if (typecode < CSG_NODE) intersect_tree(c, valid, is, root, t_min, ro, rd);
+    else intersect_node(c, valid, is, root, root, t_min, ro, rd);
     return valid;
 }
 

@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 from oxr import abi, gensteps as G, scenes
-from oxr.csg import Foundry, box3, sphere, zsphere, cylinder, cone, convexpolyhedron, phicut, scale, translate, rotate_z_to
+from oxr.csg import (Foundry, box3, sphere, zsphere, cylinder, cone, convexpolyhedron, phicut, scale, translate,
+                     rotate_z_to, contiguous, discontiguous, overlap)
 
 pytestmark = pytest.mark.gpu
 
@@ -261,6 +262,80 @@ def test_intersect_prim_bitexact(engine_mod, oracle_lib, fp64):
             assert gv.mean() > 0.05, (p, gv.mean())
 
 
+def list_zoo():
+    f = Foundry()
+    f.add_material('M', 1.0)
+    b = f.boundary('M', None, None, 'M')
+    f.begin_solid('lists')
+    prims = [
+        contiguous(box3(100.0), sphere(40.0, 60.0, 0.0, 0.0), cylinder(20.0, -90.0, 90.0)),
+        discontiguous(sphere(30.0, -80.0, 0, 0), box3(40.0), sphere(25.0, 90.0, 10.0, 0)),
+        overlap(box3(100.0), sphere(60.0), cylinder(35.0, -80.0, 80.0)),
+        sphere(70.0) & contiguous(box3(100.0), sphere(40.0, 60.0, 0.0, 0.0)),
+        (box3(120.0) - cylinder(15.0, -100.0, 100.0)) | overlap(sphere(50.0, 0, 0, 70.0), box3(60.0).moved(translate(0, 0, 60.0))),
+    ]
+    for t in prims:
+        f.add_prim(t, b)
+    f.end_solid()
+    f.add_instance(np.eye(4), 0)
+    return f, len(prims)
+
+
+@pytest.mark.parametrize('fp64', [0, 1])
+def test_list_node_prims_bitexact(engine_mod, oracle_lib, fp64):
+    """List-node prims (CSG_CONTIGUOUS / DISCONTIGUOUS / OVERLAP, alone and as tree
+    leaves) run the LIST kernel variants; intersects equal the oracle's."""
+    f, nprim = list_zoo()
+    cfg = abi.default_config(fp64_roots=fp64)
+    rng = np.random.default_rng(19)
+    o = oracle_lib.Oracle(f.arrays(), f.physics(), cfg)
+    with engine_mod.Engine(f.arrays(), f.physics(), cfg) as e:
+        for p in range(nprim):
+            rays = random_rays(rng, 4000, 150.0)
+            gi, gv = e.intersect_prim(p, rays)
+            oi, ov = o.intersect_prim(p, rays, 0)
+            assert np.array_equal(gv, ov), p
+            assert np.array_equal(bits(gi), bits(oi)), p
+            assert gv.mean() > 0.05, (p, gv.mean())
+
+
+def list_scene():
+    """Water shapes made of list nodes in air inside an absorbing box, one of them a
+    sensor, lit by an isotropic torch."""
+    f = Foundry()
+    f.add_material('Air', 1.0003, absorption_length=1e7, scattering_length=1e7)
+    f.add_material('Water', 1.333, absorption_length=2e4, scattering_length=5e3)
+    f.add_implicit_absorber('Implicit_Air_Rock')
+    f.add_sensor_surface('sensor', 0.5)
+    b_world = f.boundary('Air', None, 'Implicit_Air_Rock', 'Air')
+    b_w = f.boundary('Air', None, None, 'Water')
+    b_s = f.boundary('Air', None, 'sensor', 'Water')
+    f.begin_solid('r0')
+    f.add_prim(box3(1000.0), b_world)
+    f.add_prim(contiguous(box3(100.0).moved(translate(200, 0, 0)), sphere(60.0, 260.0, 0, 0)), b_w)
+    f.add_prim(discontiguous(sphere(40.0, -200, 100, 0), sphere(40.0, -200, -100, 0)), b_s)
+    f.add_prim(overlap(box3(150.0), sphere(90.0)).moved(translate(0, 250, 0)), b_w)
+    f.end_solid()
+    f.add_instance(np.eye(4), 0)
+    gs = G.torch([20000], pos=[(0.0, 0.0, 0.0)], wavelength=420.0, type=G.T_POINT_ISOTROPIC)
+    return f, gs
+
+
+@pytest.mark.parametrize('form', [abi.OXR_FORM_PLAIN, abi.OXR_FORM_PERSISTENT])
+def test_list_scene_simulate_bitexact(engine_mod, oracle_lib, form):
+    f, gs = list_scene()
+    cfg = abi.default_config(max_bounce=31, seq_enabled=1)
+    with engine_mod.Engine(f.arrays(), f.physics(), cfg) as e:
+        e.set_gensteps(gs)
+        st = e.launch(form)
+        ph, seq = e.photons(), e.seq()
+    r = oracle_lib.Oracle(f.arrays(), f.physics(), cfg).simulate(gs)
+    print('list scene', st['num_photon'], 'photons', st['bounce_steps'], 'steps', st['num_hit'], 'hits', flush=True)
+    assert np.array_equal(bits(ph), bits(r['photon']))
+    assert np.array_equal(seq, r['seq'])
+    assert st['num_hit'] > 0
+
+
 @pytest.mark.parametrize('name', ['nested_boxes', 'csi_calorimeter', 'instanced_csg', 'thin_shells'])
 def test_trace_bitexact(engine_mod, oracle_lib, name):
     s = scene(name)
@@ -292,3 +367,26 @@ def test_multibit_hitmask_requires_all_bits(engine_mod, oracle_lib):
     sel = (u & m) == m
     assert 0 < sel.sum() < ((u & F['SURFACE_DETECT']) != 0).sum()
     assert np.array_equal(bits(g['hit']), bits(r['photon'][sel]))
+
+
+def test_list_node_validation(engine_mod):
+    """Engine creation rejects list nodes the kernels cannot evaluate: a contiguous list
+    of more than 8 leaves (the reference's fixed enter arrays) and subs outside the prim."""
+    def make(tree):
+        f = Foundry()
+        f.add_material('M', 1.0)
+        b = f.boundary('M', None, None, 'M')
+        f.begin_solid('s')
+        f.add_prim(tree, b)
+        f.end_solid()
+        f.add_instance(np.eye(4), 0)
+        return f
+    nine = contiguous(*[sphere(10.0, 15.0 * k, 0, 0) for k in range(9)])
+    with pytest.raises(engine_mod.OxrError, match='more than 8'):
+        f = make(nine)
+        engine_mod.Engine(f.arrays(), f.physics(), abi.default_config())
+    f = make(overlap(box3(10.0), sphere(7.0)))
+    a = f.arrays()
+    a['node'].view(np.uint32).reshape(-1, 16)[0, 1] = 7        # subOffset past the prim's nodes
+    with pytest.raises(engine_mod.OxrError, match='subs outside'):
+        engine_mod.Engine(a, f.physics(), abi.default_config())
