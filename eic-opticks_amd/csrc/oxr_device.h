@@ -41,7 +41,8 @@ enum : uint32_t {
 enum { BREAK = 1, CONTINUE = 2, BOUNDARY = 3 };
 enum { CSG_ZERO = 0, CSG_NODE = 11, CSG_CONTIGUOUS = 11, CSG_DISCONTIGUOUS = 12, CSG_OVERLAP = 13, CSG_LEAF = 101, CSG_SPHERE = 101, CSG_ZSPHERE = 103,
        CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110, CSG_CONVEXPOLYHEDRON = 112,
-       CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
+       CSG_HYPERBOLOID = 117, CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
+constexpr float RT_DEFAULT_MAX = 1.e27f;   // CSG/csg_intersect_leaf_head.h:10
 
 // ------------------------------------------------------------------ Philox4x32-10
 // Counter-based generator of Salmon et al. (SC'11). Stream of photon p: key = seed,
@@ -257,6 +258,47 @@ OXD bool leaf_sphere(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
         is.y = (O.y + t * d.y) / radius;
         is.z = (O.z + t * d.z) / radius;
         is.w = t;
+    }
+    return valid;
+}
+
+// csg_intersect_leaf_hyperboloid.h:18-106: one-sheet hyperboloid x^2+y^2 = r0^2((z/zf)^2+1)
+// between planes z1 < z2 (q0 = r0, zf, z1, z2); float roots in both precision modes
+OXD bool leaf_hyperboloid(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
+    const float zero = 0.f, one = 1.f;
+    const float r0 = n.q0.x, zf = n.q0.y, z1 = n.q0.z, z2 = n.q0.w;
+    const float rr0 = r0 * r0;
+    const float z1s = z1 / zf, z2s = z2 / zf;
+    const float rr1 = rr0 * (z1s * z1s + one), rr2 = rr0 * (z2s * z2s + one);
+    const float A = -rr0 / (zf * zf);
+    const float B = -rr0;
+    const float sx = d.x, sy = d.y, sz = d.z, ox = o.x, oy = o.y, oz = o.z;
+    const float dd = sx * sx + sy * sy + A * sz * sz;
+    const float b = ox * sx + oy * sy + A * oz * sz;
+    const float c = ox * ox + oy * oy + A * oz * oz + B;
+    const float disc = b * b - dd * c;
+    float t1hyp, t2hyp, sdisc;
+    rq_roots(t1hyp, t2hyp, sdisc, dd, b, c);
+    const float h1z = oz + t1hyp * sz, h2z = oz + t2hyp * sz;
+    const float osz = one / sz;
+    const float t2cap = (z2 - oz) * osz, t1cap = (z1 - oz) * osz;
+    const f3 c1 = add(o, mul(t1cap, d)), c2 = add(o, mul(t2cap, d));
+    const float crr1 = c1.x * c1.x + c1.y * c1.y, crr2 = c2.x * c2.x + c2.y * c2.y;
+    const float tc0 = t1hyp > t_min && disc > zero && h1z > z1 && h1z < z2 ? t1hyp : RT_DEFAULT_MAX;
+    const float tc1 = t2hyp > t_min && disc > zero && h2z > z1 && h2z < z2 ? t2hyp : RT_DEFAULT_MAX;
+    const float tc2 = t2cap > t_min && crr2 < rr2 ? t2cap : RT_DEFAULT_MAX;
+    const float tc3 = t1cap > t_min && crr1 < rr1 ? t1cap : RT_DEFAULT_MAX;
+    const float t_cand = fminf(fminf(tc0, tc1), fminf(tc2, tc3));
+    const bool valid = t_cand > t_min && t_cand < RT_DEFAULT_MAX;
+    if (valid) {
+        is.w = t_cand;
+        if (t_cand == t1hyp || t_cand == t2hyp) {
+            const f3 p = add(o, mul(t_cand, d));
+            const f3 nn = normalize(mk(p.x, p.y, A * p.z));
+            is.x = nn.x; is.y = nn.y; is.z = nn.z;
+        } else {
+            is.x = zero; is.y = zero; is.z = t_cand == t1cap ? -one : one;
+        }
     }
     return valid;
 }
@@ -546,6 +588,7 @@ OXD bool intersect_leaf(Is& is, const float4* node, int ni, const float4* plan, 
         case CSG_CONVEXPOLYHEDRON: valid = leaf_convexpolyhedron(is, n, plan, t_min, o, d); break;
         case CSG_PHICUT: valid = leaf_phicut(is, n, t_min, o, d); break;
         case CSG_HALFSPACE: valid = leaf_halfspace(is, n, t_min, o, d); break;
+        case CSG_HYPERBOLOID: valid = leaf_hyperboloid(is, n, t_min, o, d); break;
         default: break;
     }
     if (valid && gt > 0) {
@@ -580,7 +623,6 @@ OXD int classify(const Is& ise, f3 dir, float tmin) {
 // first node): CSG_CONTIGUOUS / CSG_DISCONTIGUOUS / CSG_OVERLAP (CSG/csg_intersect_node.h).
 // `is` is written only for a valid intersect. Geometries without list nodes run kernels
 // built with LIST=false, which contain none of this code.
-constexpr float RT_DEFAULT_MAX = 1.e27f;
 constexpr int CONTIGUOUS_MAX_SUB = 8;   // enter/aux/idx arrays of csg_intersect_node.h:470-472
 
 // intersect_node_discontiguous (csg_intersect_node.h:655-686): closest valid sub intersect

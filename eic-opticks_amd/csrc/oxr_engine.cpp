@@ -218,7 +218,8 @@ static void invert_instance(const float* q, double W[4][3]) {
 }
 
 static bool supported_leaf(uint32_t tc) {
-    return tc == 101 || tc == 103 || tc == 105 || tc == 108 || tc == 110 || tc == 112 || tc == 121 || tc == 125;
+    return tc == 101 || tc == 103 || tc == 105 || tc == 108 || tc == 110 || tc == 112 || tc == 117 || tc == 121 ||
+           tc == 125;
 }
 
 static bool is_list(uint32_t tc) { return tc == 11 || tc == 12 || tc == 13; }

@@ -33,6 +33,7 @@ CSG_CYLINDER = 105
 CSG_CONE = 108
 CSG_BOX3 = 110
 CSG_CONVEXPOLYHEDRON = 112
+CSG_HYPERBOLOID = 117
 CSG_PHICUT = 121
 CSG_HALFSPACE = 125
 
@@ -165,6 +166,12 @@ def cone(r1, z1, r2, z2):
 def convexpolyhedron(planes, aabb):
     planes = np.asarray(planes, dtype=np.float64)
     return Node(CSG_CONVEXPOLYHEDRON, (0, 0, 0, 0, 0, 0), aabb, planes=planes)
+
+
+def hyperboloid(r0, zf, z1, z2):
+    """One-sheet hyperboloid x^2+y^2 = r0^2((z/zf)^2+1) cut by the planes z1 < z2."""
+    R = float(np.sqrt(r0 * r0 * (max(z1 * z1, z2 * z2) / (zf * zf) + 1.0)))
+    return Node(CSG_HYPERBOLOID, (r0, zf, z1, z2, 0, 0), (-R, -R, z1, R, R, z2))
 
 
 def phicut(phi0_pi, phi1_pi):

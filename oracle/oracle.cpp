@@ -61,7 +61,7 @@ enum { UNDEFINED = 0, BREAK = 1, CONTINUE = 2, BOUNDARY = 3 };   // sysrap/sflow
 enum { CSG_ZERO = 0, CSG_UNION = 1, CSG_INTERSECTION = 2, CSG_DIFFERENCE = 3, CSG_NODE = 11,
        CSG_CONTIGUOUS = 11, CSG_DISCONTIGUOUS = 12, CSG_OVERLAP = 13, CSG_LEAF = 101,
        CSG_SPHERE = 101, CSG_ZSPHERE = 103, CSG_CYLINDER = 105, CSG_CONE = 108, CSG_BOX3 = 110,
-       CSG_CONVEXPOLYHEDRON = 112, CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
+       CSG_CONVEXPOLYHEDRON = 112, CSG_HYPERBOLOID = 117, CSG_PHICUT = 121, CSG_HALFSPACE = 125 };
 
 // ---------------------------------------------------------------- Philox4x32-10
 // Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11:
@@ -218,8 +218,8 @@ inline void robust_quadratic_roots(T& t1, T& t2, T& disc, T& sdisc, T d, T b, T 
     T q = b > T(0) ? -(b + sdisc) : -(b - sdisc);
     T root1 = q / d;
     T root2 = c / q;
-    t1 = std::min(root1, root2);
-    t2 = std::max(root1, root2);
+    t1 = std::fmin(root1, root2);   // fminf/fmaxf: a NaN root yields the other
+    t2 = std::fmax(root1, root2);
 }
 // csg_robust_quadratic_roots.h:125-134
 template <typename T>
@@ -229,8 +229,8 @@ inline void robust_quadratic_roots_disqualifying(T t_min, T& t1, T& t2, T& disc,
     T q = b > T(0) ? -(b + sdisc) : -(b - sdisc);
     T root1 = sdisc > T(0) ? q / d : t_min;
     T root2 = sdisc > T(0) ? c / q : t_min;
-    t1 = std::min(root1, root2);
-    t2 = std::max(root1, root2);
+    t1 = std::fmin(root1, root2);   // fminf/fmaxf: a NaN root yields the other
+    t2 = std::fmax(root1, root2);
 }
 
 // fminf/fmaxf semantics (NaN-ignoring) for the cases that rely on them
@@ -507,6 +507,45 @@ void leaf_phicut(bool& valid, Isect<T>& is, const Node& n, T t_min, V3<T> o, V3<
     }
 }
 
+// csg_intersect_leaf_hyperboloid.h:18-106 : x^2+y^2 = r0^2((z/zf)^2+1) between z1 < z2
+template <typename T>
+void leaf_hyperboloid(bool& valid, Isect<T>& is, const Node& n, T t_min, V3<T> o, V3<T> dir) {
+    const T zero(0), one(1), tmax(1.e27f);
+    const T r0 = n.f[0], zf = n.f[1], z1 = n.f[2], z2 = n.f[3];
+    const T rr0 = r0 * r0;
+    const T z1s = z1 / zf, z2s = z2 / zf;
+    const T rr1 = rr0 * (z1s * z1s + one), rr2 = rr0 * (z2s * z2s + one);
+    const T A = -rr0 / (zf * zf);
+    const T B = -rr0;
+    const T sx = dir.x, sy = dir.y, sz = dir.z, ox = o.x, oy = o.y, oz = o.z;
+    const T d = sx * sx + sy * sy + A * sz * sz;
+    const T b = ox * sx + oy * sy + A * oz * sz;
+    const T c = ox * ox + oy * oy + A * oz * oz + B;
+    T t1hyp, t2hyp, disc, sdisc;
+    robust_quadratic_roots(t1hyp, t2hyp, disc, sdisc, d, b, c);
+    const T h1z = oz + t1hyp * sz, h2z = oz + t2hyp * sz;
+    const T osz = one / sz;
+    const T t2cap = (z2 - oz) * osz, t1cap = (z1 - oz) * osz;
+    const V3<T> c1 = o + t1cap * dir, c2 = o + t2cap * dir;
+    const T crr1 = c1.x * c1.x + c1.y * c1.y, crr2 = c2.x * c2.x + c2.y * c2.y;
+    const T tc0 = t1hyp > t_min && disc > zero && h1z > z1 && h1z < z2 ? t1hyp : tmax;
+    const T tc1 = t2hyp > t_min && disc > zero && h2z > z1 && h2z < z2 ? t2hyp : tmax;
+    const T tc2 = t2cap > t_min && crr2 < rr2 ? t2cap : tmax;
+    const T tc3 = t1cap > t_min && crr1 < rr1 ? t1cap : tmax;
+    const T t_cand = std::fmin(std::fmin(tc0, tc1), std::fmin(tc2, tc3));
+    valid = t_cand > t_min && t_cand < tmax;
+    if (valid) {
+        is.w = t_cand;
+        if (t_cand == t1hyp || t_cand == t2hyp) {
+            const V3<T> p = o + t_cand * dir;
+            const V3<T> nn = normalize(v3<T>(p.x, p.y, A * p.z));
+            is.x = nn.x; is.y = nn.y; is.z = nn.z;
+        } else {
+            is.x = zero; is.y = zero; is.z = t_cand == t1cap ? -one : one;
+        }
+    }
+}
+
 // csg_intersect_leaf_halfspace.h : p.n = w plane, inside is p.n < w; unbounded exit
 template <typename T>
 void leaf_halfspace(bool& valid, Isect<T>& is, const Node& n, T t_min, V3<T> o, V3<T> d) {
@@ -556,6 +595,7 @@ void intersect_leaf(const Ctx& c, bool& valid, Isect<T>& is, const Node& n, T t_
         case CSG_CONVEXPOLYHEDRON: leaf_convexpolyhedron(valid, is, n, c.plan, t_min, o, d); break;
         case CSG_PHICUT: leaf_phicut(valid, is, n, t_min, o, d); break;
         case CSG_HALFSPACE: leaf_halfspace(valid, is, n, t_min, o, d); break;
+        case CSG_HYPERBOLOID: leaf_hyperboloid(valid, is, n, t_min, o, d); break;
         default: break;
     }
     if (valid && q) left_multiply_normal(q, is);

@@ -5,7 +5,7 @@ import pytest
 
 from oxr import abi, gensteps as G, scenes
 from oxr.csg import (Foundry, box3, sphere, zsphere, cylinder, cone, convexpolyhedron, phicut, scale, translate,
-                     rotate_z_to, contiguous, discontiguous, overlap)
+                     rotate_z_to, contiguous, discontiguous, overlap, hyperboloid)
 
 pytestmark = pytest.mark.gpu
 
@@ -219,6 +219,7 @@ def zoo():
         cone(80.0, -60.0, 20.0, 70.0),
         convexpolyhedron(tet, (-110, -110, -110, 110, 110, 110)),
         phicut(0.25, 1.75) & sphere(90.0),
+        hyperboloid(50.0, 100.0, -100.0, 120.0),
         scenes.pmt_like_solid(),
         (sphere(80.0, 30.0, 0.0, 0.0) & box3(120.0)) - cylinder(20.0, -200.0, 200.0),
         (box3(100.0) | sphere(70.0, 0.0, 0.0, 60.0)) - (cylinder(30.0, -100.0, 100.0) | box3(20.0, 200.0, 20.0)),

@@ -108,3 +108,38 @@ def test_list_prim_layout():
     assert list(u[:, 14]) == [2, 101, 13, 110, 101]
     assert list(u[2, :2]) == [2, 3]
     assert list(a['prim'].view(np.int32).reshape(-1, 16)[0, :2]) == [5, 0]
+
+
+# ---------------------------------------------------------------- hyperboloid leaf
+def test_hyperboloid_closed_form(oracle_lib):
+    """x^2+y^2 = r0^2((z/zf)^2+1), r0=50, zf=100, between z=-100 and z=150
+    (csg_intersect_leaf_hyperboloid.h)."""
+    from oxr.csg import hyperboloid
+    H = _prim_oracle(oracle_lib, hyperboloid(50.0, 100.0, -100.0, 150.0))
+    A = -50.0 ** 2 / 100.0 ** 2
+    r100 = np.sqrt(2500.0 * 2.0)
+    rays = _rays([[-300, 0, 0], [-300, 0, 100], [0, 0, -400], [0, 0, 400]],
+                 [[1, 0, 0], [1, 0, 0], [0, 0, 1], [0, 0, -1]])
+    for precision in (0, 1):
+        i, v = H.intersect_prim(0, rays, precision)
+        assert v.all()
+        assert np.allclose(i[0], [-1, 0, 0, 250], atol=1e-4)            # waist radius
+        n = np.array([-r100, 0.0, A * 100.0]); n /= np.linalg.norm(n)
+        assert np.allclose(i[1, :3], n, atol=1e-5) and abs(i[1, 3] - (300 - r100)) < 1e-3
+        assert np.allclose(i[2], [0, 0, -1, 300], atol=1e-4)            # lower cap
+        assert np.allclose(i[3], [0, 0, 1, 250], atol=1e-4)             # upper cap
+
+
+@pytest.mark.parametrize('from_inside', [False, True])
+def test_hyperboloid_matches_brute_force(oracle_lib, from_inside):
+    from oxr.csg import hyperboloid
+    O = _prim_oracle(oracle_lib, hyperboloid(50.0, 100.0, -100.0, 150.0))
+
+    def inside(p):
+        return (p[:, 0] ** 2 + p[:, 1] ** 2 < 2500.0 * ((p[:, 2] / 100.0) ** 2 + 1.0)) & (p[:, 2] > -100.0) & (p[:, 2] < 150.0)
+    o, d = _rays_through(inside, 160.0, 120, 13, from_inside)
+    want = _first_crossing(inside, o, d, 1000.0)
+    isect, valid = O.intersect_prim(0, _rays(o, d), 1)
+    got = np.where(valid.astype(bool), isect[:, 3], np.nan)
+    agree = (np.isnan(want) & np.isnan(got)) | (np.abs(got - want) < 2e-3)
+    assert agree.mean() >= 0.99, (agree.mean(), np.c_[want, got][~agree][:5])
