@@ -5,6 +5,7 @@ import ctypes as C
 
 import numpy as np
 
+OXR_OK, OXR_EINVAL, OXR_ENOMEM, OXR_EHIP, OXR_ECAPACITY, OXR_ESTATE, OXR_ETIMEOUT, OXR_ENOTFOUND = 0, -1, -2, -3, -4, -5, -6, -7
 OXR_FORM_PLAIN = 0
 OXR_FORM_PERSISTENT = 1
 OXR_RNG_PRODUCTION = 0

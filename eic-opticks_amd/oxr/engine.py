@@ -70,6 +70,9 @@ def lib():
             'oxr_server_wait': (i32, [vp, P(abi.RequestResult), i32]),
             'oxr_server_take_hits': (i32, [vp, i64, vp, i64, P(i64)]),
             'oxr_server_stats': (i32, [vp, vp]),
+            'oxr_foundry_load': (i32, [C.c_char_p, P(vp)]),
+            'oxr_foundry_arrays': (i32, [vp, P(abi.Geometry), P(abi.Physics)]),
+            'oxr_foundry_free': (None, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

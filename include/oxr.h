@@ -43,16 +43,16 @@ enum {
     OXR_ECAPACITY     = -4,   /* caller buffer too small (count still returned)   */
     OXR_ESTATE        = -5,   /* call out of order (e.g. launch before gensteps)  */
     OXR_ETIMEOUT      = -6,   /* server wait timed out                            */
-    OXR_ENOTFOUND     = -7    /* unknown request id                               */
+    OXR_ENOTFOUND     = -7    /* unknown request id / missing CSGFoundry file     */
 };
 
-/* ---- engine forms (north star: plain per-launch, persistent lane-refill) */
+/* ---- engine forms: plain per-launch, persistent lane-refill ----------- */
 enum {
     OXR_FORM_PLAIN      = 0,  /* one lane per photon, CSGOptiX7.cu:405-503 shape    */
     OXR_FORM_PERSISTENT = 1   /* persistent grid, lanes refill from a device queue  */
 };
 
-/* ---- random-number consumption schedules (SURVEY: DEBUG_TAG vs PRODUCTION) */
+/* ---- random-number consumption schedules: DEBUG_TAG vs PRODUCTION ----- */
 enum {
     OXR_RNG_PRODUCTION = 0,   /* qsim.h built with PRODUCTION: no alignment burns     */
     OXR_RNG_ALIGNED    = 1    /* qsim.h built !PRODUCTION + DEBUG_TAG (reference's
@@ -104,7 +104,8 @@ typedef struct oxr_physics {
     int32_t         icdf_rows;   /* 1 or 3                                             */
     int32_t         icdf_width;  /* 4096                                               */
     int32_t         hd_factor;   /* 0, 10 or 20 (qudarap/qscint.h:150-228)             */
-    float           reemission_scint_time; /* ns, reemit time constant (0 = none)      */
+    float           reemission_scint_time; /* reserved, ignored: re-emission adds no delay
+                                              (qsim.h:804-825)                           */
 } oxr_physics;
 
 /* Stepping and event controls (CSGOptiX/Params.h:29-122, sysrap/SEventConfig.cc). */
@@ -187,6 +188,18 @@ int  oxr_engine_get_tuning(oxr_engine* e, oxr_tuning* t);
 int  oxr_engine_query(oxr_engine* e, oxr_engine_info* info);
 /* Fill *cfg with the reference defaults (SEventConfig.cc:45-106). */
 void oxr_config_defaults(oxr_config* cfg);
+
+/* ---- CSGFoundry folders -------------------------------------------------
+ * Replaces CSGFoundry::Load (CSG/CSGFoundry.cc:2889-3020) with SSim::load
+ * (sysrap/SSim.cc:486-520): reads <cfbase>/CSGFoundry (or cfbase itself when it holds
+ * solid.npy) - solid, prim, node, plan, itra, inst .npy and SSim/stree/standard bnd,
+ * optical, icdf .npy with the bnd_meta.txt wavelength domain - on the host. No device
+ * calls. */
+typedef struct oxr_foundry oxr_foundry;
+int  oxr_foundry_load(const char* cfbase, oxr_foundry** out);
+/* Views of the loaded arrays for oxr_engine_create, valid until oxr_foundry_free. */
+int  oxr_foundry_arrays(const oxr_foundry* f, oxr_geometry* geo, oxr_physics* phys);
+void oxr_foundry_free(oxr_foundry* f);
 
 /* ---- per-event state ---------------------------------------------------
  * Replaces QEvt::setGenstepUpload_NP (qudarap/QEvt.cc:199-256,332-445) and
