@@ -1,5 +1,5 @@
 // oxr_foundry.cpp -- reads a CSGFoundry folder as the reference persists it
-// (CSGFoundry::save_, CSG/CSGFoundry.cc:2768-2800; CSGFoundry::load :2889-3020; SSim under
+// (CSGFoundry::save_, CSG/CSGFoundry.cc:2768-2800; CSGFoundry::load :2977-3050, loadArray :3324-3350; SSim under
 // SSim/stree/standard, sysrap/SSim.hh:52-55) into host arrays that oxr_engine_create takes.
 // Host only: no device calls, so it runs and is tested without a GPU.
 //

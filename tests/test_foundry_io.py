@@ -1,5 +1,5 @@
 """CSGFoundry folders in the reference layout (CSG/CSGFoundry.cc:2768-2800 save_,
-:2889-3020 load; SSim tables under SSim/stree/standard, sysrap/SSim.hh:52-55): a scene
+:2977-3050 load; SSim tables under SSim/stree/standard, sysrap/SSim.hh:52-55): a scene
 written with oxr.cfio.save_foundry and read back through the library's own NPY reader
 (oxr_foundry_load, host only) gives the same arrays, the same oracle photons, and clear
 errors for folders the reader cannot take."""
