@@ -233,6 +233,32 @@ def _positivize(n: Node, neg=False) -> Node:
     return Node(op, left=l, right=r)
 
 
+MAX_TREE_HEIGHT = 7      # complete binary tree of 255 nodes, the largest a prim may hold
+
+
+def _operands(n: Node, op):
+    if n.left is not None and n.typecode == op:
+        return _operands(n.left, op) + _operands(n.right, op)
+    return [n]
+
+
+def _balanced(n: Node) -> Node:
+    """Rebuild each chain of one operator as a balanced tree. Applied to positivised
+    trees only, where every operator is a union or an intersection (associative and
+    commutative), so the solid is unchanged; a chain of k operands drops from height
+    k-1 to ceil(log2 k). Long subtraction chains (a plate minus many holes) need it."""
+    if n.left is None:
+        return n
+    items = [_balanced(x) for x in _operands(n, n.typecode)]
+
+    def build(xs):
+        if len(xs) == 1:
+            return xs[0]
+        h = len(xs) // 2
+        return Node(n.typecode, left=build(xs[:h]), right=build(xs[h:]))
+    return build(items)
+
+
 def _leaf_aabb(n: Node):
     return n.aabb if n.transform is None else transform_aabb(n.aabb, n.transform)
 
@@ -285,29 +311,43 @@ class Foundry:
 
     # -- materials / surfaces ---------------------------------------------
     def add_material(self, name, rindex, absorption_length=1e6, scattering_length=1e6,
-                     reemission_prob=0.0):
+                     reemission_prob=0.0, group_velocity=None):
+        """Material rows over the wavelength domain; each property is a constant, a
+        function of wavelength (nm) or an array over the domain. Without group_velocity
+        it is derived from rindex as c/(n - lambda dn/dlambda)."""
         wl = domain()
 
         def arr(v):
             if callable(v):
                 return np.asarray(v(wl), dtype=np.float64)
+            if np.ndim(v):
+                a = np.asarray(v, dtype=np.float64)
+                assert a.shape == (NUM_WL,), a.shape
+                return a
             return np.full(NUM_WL, float(v))
 
         n = arr(rindex)
-        dn = np.gradient(n, wl)
-        vg = C_LIGHT / (n - wl * dn)          # group velocity c/(n - lambda dn/dlambda)
+        if group_velocity is None:
+            dn = np.gradient(n, wl)
+            vg = C_LIGHT / (n - wl * dn)
+        else:
+            vg = arr(group_velocity)
         self.materials[name] = dict(rindex=n, abs=arr(absorption_length),
                                     ray=arr(scattering_length), reemit=arr(reemission_prob),
                                     vg=vg)
         self.mat_order.append(name)
 
-    def add_surface(self, name, detect=0.0, absorb=0.0, specular=0.0, diffuse=0.0, osn0='X'):
+    def add_surface(self, name, detect=0.0, absorb=0.0, specular=0.0, diffuse=0.0, osn0='X',
+                    finish=0, model_value_percent=0):
         """Surface payload as U4SurfaceArray::addSurface: sensor -> (eff, 1-eff, 0, 0);
-        polished -> (0, 1-R, R, 0); ground -> (0, 1-R, 0, R)."""
+        polished -> (0, 1-R, R, 0); ground -> (0, 1-R, 0, R). Each component is a constant
+        or an array over the wavelength domain. finish and model_value_percent fill the
+        optical row's z and w (sstandard::make_optical, sysrap/sstandard.h:262-380)."""
         p = np.zeros((NUM_WL, 4))
-        p[:] = (detect, absorb, specular, diffuse)
-        assert abs(p[0].sum() - 1.0) < 1e-9, name
-        self.surfaces[name] = (p, osn0)
+        for k, v in enumerate((detect, absorb, specular, diffuse)):
+            p[:, k] = v
+        assert np.all(np.abs(p.sum(axis=1) - 1.0) < 1e-9), name
+        self.surfaces[name] = (p, osn0, int(finish), int(model_value_percent))
         self.sur_order.append(name)
 
     def add_sensor_surface(self, name, efficiency, osn0='X'):
@@ -315,7 +355,7 @@ class Foundry:
 
     def add_implicit_absorber(self, name):
         """U4SurfaceArray::addImplicit: perfect absorber for RINDEX-less materials."""
-        self.add_surface(name, absorb=1.0, osn0='X')
+        self.add_surface(name, absorb=1.0, osn0='X', finish=1, model_value_percent=100)
 
     def boundary(self, omat, osur, isur, imat):
         key = (omat, osur, isur, imat)
@@ -334,7 +374,11 @@ class Foundry:
     def add_prim(self, tree: Node, boundary: int, mesh_idx=0):
         assert self._cur_solid is not None
         t = _positivize(tree)
+        if t.height() > MAX_TREE_HEIGHT:
+            t = _balanced(t)
         h = t.height()
+        if h > MAX_TREE_HEIGHT:
+            raise ValueError(f'CSG tree of height {h} exceeds {MAX_TREE_HEIGHT} after balancing')
         num = (1 << (h + 1)) - 1
         flat = [None] * num
 
@@ -483,11 +527,11 @@ class Foundry:
                     if name is None:
                         optical[line] = (0, SMATSUR_NOSURFACE, 0, 0)
                         continue
-                    payload, osn0 = self.surfaces[name]
+                    payload, osn0, finish, mvp = self.surfaces[name]
                     bnd[b, j, 0] = payload
                     bnd[b, j, 1] = 0.0
                     ems = {'-': SMATSUR_NOSURFACE, '#': SMATSUR_SENSOR_A}.get(osn0, SMATSUR_SURFACE)
-                    optical[line] = (self.sur_order.index(name) + 1, ems, 0, 0)
+                    optical[line] = (self.sur_order.index(name) + 1, ems, finish, mvp)
         return dict(bnd=bnd.astype(np.float32), optical=optical,
                     wl_min=WL_MIN, wl_step=WL_STEP)
 

@@ -33,7 +33,7 @@ def _blank(n):
 
 def torch(numphoton, pos=(0, 0, 0), mom=(0, 0, 1), pol=(1, 0, 0), wavelength=420.0,
           type=T_POINT_ISOTROPIC, radius=0.0, distance=0.0, zenith=(0.0, 1.0),
-          azimuth=(0.0, 1.0), time=0.0, matline=0, trackid=0):
+          azimuth=(0.0, 1.0), time=0.0, matline=0, trackid=0, weight=1.0, mode=255):
     """One torch genstep per entry of numphoton (scalar or array); pos may be (n,3)."""
     numphoton = np.atleast_1d(np.asarray(numphoton, dtype=np.int64))
     n = len(numphoton)
@@ -45,14 +45,14 @@ def torch(numphoton, pos=(0, 0, 0), mom=(0, 0, 1), pol=(1, 0, 0), wavelength=420
     gs[:, 1, :3] = np.broadcast_to(np.asarray(pos, dtype=np.float32), (n, 3))
     gs[:, 1, 3] = time
     gs[:, 2, :3] = np.broadcast_to(np.asarray(mom, dtype=np.float32), (n, 3))
-    gs[:, 2, 3] = 1.0
+    gs[:, 2, 3] = weight
     gs[:, 3, :3] = np.broadcast_to(np.asarray(pol, dtype=np.float32), (n, 3))
     gs[:, 3, 3] = wavelength
     gs[:, 4, 0:2] = zenith
     gs[:, 4, 2:4] = azimuth
     gs[:, 5, 0] = radius
     gs[:, 5, 1] = distance
-    u[:, 5, 2] = 255
+    u[:, 5, 2] = mode
     u[:, 5, 3] = type
     return gs
 
