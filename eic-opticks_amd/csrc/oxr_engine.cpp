@@ -818,6 +818,51 @@ int oxr_trace_rays(oxr_engine* eh, const float* rays, int64_t n, float* prd) {
     return OXR_OK;
 }
 
+int oxr_simtrace(oxr_engine* eh, const float* gs, int64_t num_gs, float* simtrace, int64_t capacity, int64_t* num_out) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !num_out || num_gs < 0 || capacity < 0 || (num_gs > 0 && !gs)) return fail(OXR_EINVAL, "bad argument");
+    *num_out = 0;
+    std::vector<long long> off(size_t(num_gs) + 1, 0);
+    for (int64_t g = 0; g < num_gs; g++) {
+        const float* q = gs + 24 * g;
+        const uint32_t code = fbits(q[0]), axes = fbits(q[1]);
+        const int32_t np = int32_t(fbits(q[3]));
+        if (code != 17) return fail(OXR_EINVAL, "simtrace takes FRAME gensteps (gencode 17) only");
+        if (axes > 3u) return fail(OXR_EINVAL, "FRAME genstep grid axes must be XYZ, YZ, XZ or XY (0-3)");
+        if (np < 0) return fail(OXR_EINVAL, "FRAME genstep with a negative photon count");
+        off[size_t(g) + 1] = off[size_t(g)] + np;
+    }
+    const int64_t n = off[size_t(num_gs)];
+    *num_out = n;
+    if (n > capacity) return fail(OXR_ECAPACITY, "simtrace buffer too small");
+    if (n == 0) return OXR_OK;
+    if (!simtrace) return fail(OXR_EINVAL, "null simtrace buffer");
+    if (num_gs > 0x7fffffffLL) return fail(OXR_EINVAL, "too many gensteps");
+    std::lock_guard<std::mutex> lk(e->mu);
+    DBuf<float4> dg, dout;
+    DBuf<long long> doff;
+    auto cleanup = [&]() { dg.release(); dout.release(); doff.release(); };
+    hipError_t x;
+    if ((x = dg.reserve(size_t(num_gs) * 6)) != hipSuccess || (x = doff.reserve(size_t(num_gs) + 1)) != hipSuccess ||
+        (x = dout.reserve(size_t(n) * 4)) != hipSuccess) {
+        cleanup(); return hip_fail(x, "hipMalloc simtrace");
+    }
+    if ((x = hipMemcpy(dg.p, gs, size_t(num_gs) * 96, hipMemcpyHostToDevice)) != hipSuccess ||
+        (x = hipMemcpy(doff.p, off.data(), off.size() * sizeof(long long), hipMemcpyHostToDevice)) != hipSuccess) {
+        cleanup(); return hip_fail(x, "copy FRAME gensteps");
+    }
+    EventBuffers dummy;
+    oxd::Params P = e->params(dummy, 0, 0, 0, e->ntop_for(256));
+    if ((x = oxl::simtrace(e->stream, P, dg.p, doff.p, int(num_gs), n, dout.p, e->cfg.fp64_roots != 0, e->has_list)) != hipSuccess) {
+        cleanup(); return hip_fail(x, "simtrace launch");
+    }
+    if ((x = hipStreamSynchronize(e->stream)) != hipSuccess) { cleanup(); return hip_fail(x, "simtrace sync"); }
+    x = hipMemcpy(simtrace, dout.p, size_t(n) * 64, hipMemcpyDeviceToHost);
+    cleanup();
+    if (x != hipSuccess) return hip_fail(x, "copy simtrace");
+    return OXR_OK;
+}
+
 int oxr_intersect_prim(oxr_engine* eh, int32_t prim_index, const float* rays, int64_t n, float* isect, int32_t* valid) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || n < 0 || (n > 0 && (!rays || !isect || !valid))) return fail(OXR_EINVAL, "bad argument");

@@ -79,6 +79,8 @@ hipError_t merge(hipStream_t s, bool lite, const void* in, long long n, unsigned
                  float time_window, void* out, long long* num_out);
 hipError_t trace_rays(hipStream_t s, const oxd::Params& P, const float4* rays, long long n, float4* prd, bool fp64,
                       bool list);
+hipError_t simtrace(hipStream_t s, const oxd::Params& P, const float4* gs, const long long* offsets, int num_gs,
+                    long long n, float4* out, bool fp64, bool list);
 hipError_t intersect_prim(hipStream_t s, const oxd::Geom& g, int root, const float4* rays, long long n,
                           float4* isect, int* valid, bool fp64, bool list);
 } // namespace oxl

@@ -61,6 +61,7 @@ def lib():
             'oxr_merge_hits': (i32, [vp, i32, C.c_float, vp, i64, vp, i64, P(i64)]),
             'oxr_simulate': (i32, [vp, vp, i64, i32, i32, vp, i64, P(i64), P(abi.LaunchStats)]),
             'oxr_trace_rays': (i32, [vp, vp, i64, vp]),
+            'oxr_simtrace': (i32, [vp, vp, i64, vp, i64, P(i64)]),
             'oxr_intersect_prim': (i32, [vp, i32, vp, i64, vp, vp]),
             'oxr_autotune': (i32, [vp, i32, vp, i32, P(i32)]),
             'oxr_server_create': (i32, [vp, P(abi.ServerConfig), P(vp)]),
@@ -249,6 +250,17 @@ class Engine:
         prd = np.zeros((len(rays), 2, 4), dtype=np.float32)
         _check(lib().oxr_trace_rays(self.h, abi.ptr(rays), len(rays), abi.ptr(prd)))
         return prd
+
+    def simtrace(self, gensteps):
+        """2-D cross-section rays of FRAME gensteps as the (n,4,4) simtrace array."""
+        gs = np.ascontiguousarray(gensteps, dtype=np.float32).reshape(-1, 6, 4)
+        n = C.c_int64(0)
+        rc = lib().oxr_simtrace(self.h, abi.ptr(gs), len(gs), None, 0, C.byref(n))
+        if rc not in (0, abi.OXR_ECAPACITY):
+            _check(rc)
+        out = np.zeros((n.value, 4, 4), dtype=np.float32)
+        _check(lib().oxr_simtrace(self.h, abi.ptr(gs), len(gs), abi.ptr(out), n.value, C.byref(n)))
+        return out
 
     def intersect_prim(self, prim, rays):
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 2, 4)

@@ -17,6 +17,7 @@ GS_SCINT_1042 = 2
 GS_TORCH = 6
 GS_CERENKOV = 15
 GS_SCINTILLATION = 16
+GS_FRAME = 17
 GS_INPUT_PHOTON = 19
 
 # storch types (sysrap/storchtype.h:10-17); T_POINT_ISOTROPIC is an extension of
@@ -120,3 +121,68 @@ def input_photon(numphoton):
 
 def num_photons(gs):
     return int(np.asarray(gs).view(np.uint32)[:, 0, 3].astype(np.int64).sum())
+
+
+# grid axes of FRAME gensteps (sysrap/sxyz.h)
+AX_XYZ, AX_YZ, AX_XZ, AX_XY = range(4)
+
+
+def grid_axes(nx, ny, nz):
+    """SGenstep::GridAxes: a planar grid (one dimension zero) names its plane."""
+    if nx == 0 and ny > 0 and nz > 0:
+        return AX_YZ
+    if nx > 0 and ny == 0 and nz > 0:
+        return AX_XZ
+    if nx > 0 and ny > 0 and nz == 0:
+        return AX_XY
+    return AX_XYZ
+
+
+def genstep_id(ix, iy, iz, iw=0):
+    """SGenstep::GenstepID: four signed chars packed little-endian into 32 bits."""
+    return int(np.array([ix, iy, iz, iw], dtype=np.int8).view(np.uint32)[0])
+
+
+def frame(ce, cegs, gridscale=1.0, geotran=None, offsets=((0.0, 0.0, 0.0),), ce_scale=True):
+    """FRAME gensteps on a centre-extent grid, SFrameGenstep::MakeCenterExtentGenstep
+    (sysrap/SFrameGenstep.cc:604-700). ce = (cx, cy, cz, extent); cegs = (nx, ny, nz,
+    photons) for the symmetric grid -n..n, or the standardised
+    (ix0, ix1, iy0, iy1, iz0, iz1, photons, high). Each genstep carries the transform
+    geotran x translate(grid point * extent * gridscale / high) as a qat4 in q2..q5;
+    geotran defaults to a translation to the centre (row-convention 4x4)."""
+    cegs = list(cegs)
+    if len(cegs) == 4:
+        nx, ny, nz, ph = cegs
+        cegs = [-nx, nx, -ny, ny, -nz, nz, ph, 1]
+    if len(cegs) != 8:
+        raise ValueError('cegs has 4 or 8 entries')
+    ix0, ix1, iy0, iy1, iz0, iz1, ph, high = cegs
+    if not 1 <= high <= 8:
+        raise ValueError('cegs high must be 1..8')
+    scale = float(gridscale) / float(high)
+    ix0, ix1, iy0, iy1, iz0, iz1 = (v * high for v in (ix0, ix1, iy0, iy1, iz0, iz1))
+    axes = grid_axes((ix1 - ix0) // 2, (iy1 - iy0) // 2, (iz1 - iz0) // 2)
+    local_scale = scale * float(ce[3]) if ce_scale else scale
+    if geotran is None:
+        geotran = np.eye(4)
+        geotran[3, :3] = ce[:3]
+    geotran = np.asarray(geotran, dtype=np.float64)
+    out = []
+    for ip, off in enumerate(offsets):
+        for ix in range(ix0, ix1 + 1):
+            for iy in range(iy0, iy1 + 1):
+                for iz in range(iz0, iz1 + 1):
+                    shift = np.eye(4)
+                    shift[3, :3] = (ix * local_scale, iy * local_scale, iz * local_scale)
+                    m = shift @ geotran                      # grid shift first, then geotran
+                    gs = np.zeros((6, 4), dtype=np.float32)
+                    u = gs.view(np.uint32)
+                    u[0, 0] = GS_FRAME
+                    u[0, 1] = axes
+                    u[0, 2] = genstep_id(ix, iy, iz, ip)
+                    u[0, 3] = ph
+                    gs[1, :3] = off
+                    gs[1, 3] = 1.0
+                    gs[2:6] = m.astype(np.float32)
+                    out.append(gs)
+    return np.asarray(out, dtype=np.float32).reshape(-1, 6, 4)

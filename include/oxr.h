@@ -265,6 +265,20 @@ int oxr_simulate(oxr_engine* e, const float* gs, int64_t num_gs, int32_t event_i
  * prd out (n,2,4) = quad2 {normal(world, unnormalised), t}, {lposcost, lposfphi,
  * iindex<<16|identity, globalPrimIdx<<16|boundary}; miss -> boundary 0xffff. */
 int oxr_trace_rays(oxr_engine* e, const float* rays, int64_t n, float* prd);
+
+/* ---- simtrace: 2-D geometry cross-sections -----------------------------
+ * QSim::simtrace (qudarap/QSim.cc:857-890) launching the simtrace raygen
+ * (CSGOptiX7.cu:536-577) over FRAME gensteps (gencode 17, q0 = {17, grid axes 0-3 =
+ * XYZ/YZ/XZ/XY, genstep id, num rays}, q1 = local start position, q2..q5 = qat4
+ * transform; SFrameGenstep::MakeCenterExtentGenstep, sysrap/SFrameGenstep.cc:604-700).
+ * Each ray's direction comes from two uniforms of its photon-index stream
+ * (qsim::generate_photon_simtrace_frame, qsim.h:2459-2510). Output (n,4,4) f32 as
+ * sevent::add_simtrace (sysrap/sevent.h:670-696): {normal, t}, {pos + t dir, tmin},
+ * {pos, globalPrimIdx<<16|boundary}, {dir, iindex<<16|identity}; a miss has t 0 and
+ * 0xffffffff words. *num_out is the total ray count; OXR_ECAPACITY (nothing written)
+ * when it exceeds capacity. */
+int oxr_simtrace(oxr_engine* e, const float* gensteps, int64_t num_gensteps, float* simtrace,
+                 int64_t capacity, int64_t* num_out);
 /* Single-prim intersect_prim in the prim's own frame (CSG/csg_intersect_tree.h:684-719,
  * the CSGQuery::intersect contract CSG/CSGQuery.cc:231-276): rays (n,2,4) as above,
  * isect out (n,4) = normal.xyz (object frame), t; t=0 and valid=0 on miss. */

@@ -1692,6 +1692,51 @@ void orc_trace_rays(void* cv, const float* rays, int64_t n, float* prd, int32_t 
     }
 }
 
+// simtrace over FRAME gensteps: qsim::generate_photon_simtrace_frame (qudarap/qsim.h:2459-2510)
+// then trace and sevent::add_simtrace (sysrap/sevent.h:670-696); out (n,4,4). Returns n.
+int64_t orc_simtrace(void* cv, const float* gs, int64_t num_gs, float* out, int64_t cap) {
+    const orc::Ctx& c = *static_cast<orc::Ctx*>(cv);
+    int64_t total = orc_fill_seed(gs, num_gs, nullptr, 0);
+    if (!out || total > cap) return total;
+    std::vector<int32_t> seed(static_cast<size_t>(total));
+    orc_fill_seed(gs, num_gs, seed.data(), total);
+    for (int64_t i = 0; i < total; i++) {
+        const float* q = gs + 24 * size_t(seed[size_t(i)]);
+        const uint32_t gridaxes = orc::f2u(q[1]);
+        orc::Stream rng;
+        rng.init(c.cfg.rng_seed, 0, 0, 0, uint64_t(i));
+        float u0 = rng.next();
+        float ph = 2.f * orc::pi<float>() * u0;
+        float sinPhi = orc::xsin(ph), cosPhi = orc::xcos(ph);
+        float u1 = rng.next();
+        float cosTheta = 2.f * u1 - 1.f;
+        float sinTheta = std::sqrt(1.f - cosTheta * cosTheta);
+        float d[3];
+        switch (gridaxes) {
+            case 1: d[0] = 0.f; d[1] = cosPhi; d[2] = sinPhi; break;
+            case 2: d[0] = cosPhi; d[1] = 0.f; d[2] = sinPhi; break;
+            case 3: d[0] = cosPhi; d[1] = sinPhi; d[2] = 0.f; break;
+            default: d[0] = sinTheta * cosPhi; d[1] = sinTheta * sinPhi; d[2] = cosTheta; break;
+        }
+        const float* m = q + 8;          // qat4 rows q2..q5, translation in the last
+        const float p[3] = {q[4], q[5], q[6]};
+        float o[3], v[3];
+        for (int k = 0; k < 3; k++) {
+            o[k] = m[k] * p[0] + m[4 + k] * p[1] + m[8 + k] * p[2] + m[12 + k] * 1.f;
+            v[k] = m[k] * d[0] + m[4 + k] * d[1] + m[8 + k] * d[2] + m[12 + k] * 0.f;
+        }
+        orc::PRD<float> r;
+        orc::trace<float>(c, orc::v3<float>(o[0], o[1], o[2]), orc::v3<float>(v[0], v[1], v[2]), c.cfg.tmin, c.cfg.tmax, r);
+        const float t = r.t;
+        float* a = out + 16 * size_t(i);
+        a[0] = r.normal.x; a[1] = r.normal.y; a[2] = r.normal.z; a[3] = t;
+        a[4] = o[0] + t * v[0]; a[5] = o[1] + t * v[1]; a[6] = o[2] + t * v[2]; a[7] = c.cfg.tmin;
+        a[8] = o[0]; a[9] = o[1]; a[10] = o[2]; a[11] = orc::u2f(r.gp_boundary);
+        a[12] = v[0]; a[13] = v[1]; a[14] = v[2]; a[15] = orc::u2f(r.iindex_identity);
+    }
+    return total;
+}
+
 // Full simulate of one event slice. seq (n,4) u64 may be null; record may be null.
 // photon_begin/photon_end select a sub-range of the slice (for sampling); nthreads std::threads.
 int64_t orc_simulate(void* cv, const float* gs, int64_t num_gs, int32_t event_index, uint64_t slot_offset,

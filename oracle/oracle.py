@@ -42,6 +42,8 @@ def lib():
         L.orc_fill_seed.argtypes = [vp, i64, vp, i64]
         L.orc_intersect_prim.argtypes = [vp, i32, vp, i64, vp, vp, i32]
         L.orc_trace_rays.argtypes = [vp, vp, i64, vp, i32]
+        L.orc_simtrace.argtypes = [vp, vp, i64, vp, i64]
+        L.orc_simtrace.restype = i64
         L.orc_simulate.restype = i64
         L.orc_simulate.argtypes = [vp, vp, i64, i32, u64, vp, i32, vp, i32, i32, i64, i64,
                                    vp, vp, vp, i32, vp, vp, vp]
@@ -192,6 +194,13 @@ class Oracle:
         prd = np.zeros((len(rays), 2, 4), dtype=np.float32)
         lib().orc_trace_rays(self.h, P(rays), len(rays), P(prd), precision)
         return prd
+
+    def simtrace(self, gs):
+        gs = np.ascontiguousarray(gs, dtype=np.float32).reshape(-1, 6, 4)
+        n = lib().orc_simtrace(self.h, P(gs), len(gs), None, 0)
+        out = np.zeros((n, 4, 4), dtype=np.float32)
+        lib().orc_simtrace(self.h, P(gs), len(gs), P(out), n)
+        return out
 
     def simulate(self, gs, event_index=0, slot_offset=0, precooked=None, input_photons=None,
                  precision=0, nthreads=8, begin=0, end=-1, seq=True, max_record=0, lite=False):
