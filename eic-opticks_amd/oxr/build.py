@@ -44,9 +44,28 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f'link failed:\n{r.stderr}')
     os.replace(tmp, LIB)
+    build_examples()
     if verbose:
         print('built', LIB, file=sys.stderr)
     return LIB
+
+
+EXAMPLES = {'photon_file_source': 'photon_file_source.cpp'}
+
+
+def build_examples():
+    """Host programs over the C ABI and the C++ facade, linked against liboxr.so with an
+    rpath to the package directory."""
+    inc = os.path.join(os.path.dirname(os.path.dirname(PKG)), 'include')
+    src_dir = os.path.join(os.path.dirname(PKG), 'examples')
+    for name, src in EXAMPLES.items():
+        out = os.path.join(PKG, name)
+        cmd = ['g++', '-O2', '-std=c++17', '-Wall', '-I', inc, os.path.join(src_dir, src), '-o', out + '.tmp',
+               '-L', PKG, '-loxr', '-Wl,-rpath,$ORIGIN']
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f'g++ failed for {src}:\n{r.stderr}')
+        os.replace(out + '.tmp', out)
 
 
 if __name__ == '__main__':

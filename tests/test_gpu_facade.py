@@ -1,0 +1,58 @@
+"""tests/test_GPUPhotonFileSource.sh replayed against the C++ example application
+(eic-opticks_amd/examples/photon_file_source.cpp over include/oxr_g4cx.hpp) on the GPU:
+the raindrop GDML as a CSGFoundry folder, the reference script's photon files, and its
+checks on the printed counts and the hit listing."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oxr import cfio, gdml
+
+from test_facade import EXE, GOLDEN
+from test_gdml import geom
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def raindrop_folder(tmp_path_factory):
+    d = tmp_path_factory.mktemp('raindrop_cf')
+    g = gdml.load_gdml(geom('opticks_raindrop'))
+    cfio.save_foundry(str(d), g.geometry, g.physics)
+    return str(d)
+
+
+def run(args, cwd):
+    print('photon_file_source', ' '.join(args), flush=True)
+    env = dict(os.environ, OPTICKS_MAX_SLOT='M1')
+    return subprocess.run([EXE] + args, capture_output=True, text=True, timeout=120, cwd=cwd, env=env)
+
+
+def field(out, key):
+    lines = [l for l in out.splitlines() if key in l]
+    return lines[-1].split()[-1] if lines else None
+
+
+def test_ten_photons_ten_hits(raindrop_folder, tmp_path):
+    r = run(['-g', raindrop_folder, '-p', os.path.join(GOLDEN, 'raindrop_photons.txt'), '-s', '42'], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    loaded = [l for l in r.stdout.splitlines() if l.startswith('Loaded')][0].split()[1]
+    assert loaded == '10'
+    assert field(r.stdout, 'Opticks: NumHits:') == '10'
+    lines = (tmp_path / 'opticks_hits_output.txt').read_text().splitlines()
+    assert len(lines) == 10
+    n = {w: sum(f' {w} ' in l for l in lines) for w in (420, 450, 500)}
+    assert n == {420: 3, 450: 2, 500: 5}
+
+
+def test_comments_and_blank_lines(raindrop_folder, tmp_path):
+    r = run(['-g', raindrop_folder, '-p', os.path.join(GOLDEN, 'raindrop_photons_comments.txt'), '-s', '42'], str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert [l for l in r.stdout.splitlines() if l.startswith('Loaded')][0].split()[1] == '2'
+
+
+def test_missing_photons_argument_fails(raindrop_folder, tmp_path):
+    r = run(['-g', raindrop_folder], str(tmp_path))
+    assert r.returncode != 0
