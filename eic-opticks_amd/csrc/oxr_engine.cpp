@@ -49,7 +49,20 @@ int Engine::ntop_for(int block) const {
     return int(std::min<size_t>(size_t(want), room));
 }
 
-oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int ntop) const {
+// The distance-carrying stack (8 B per entry) is used when it fits, with the wanted top
+// nodes, in a block's share of the CU's 160 KB of LDS at 4 waves per SIMD (16 waves, 1024
+// lanes per CU): more LDS per block would cost occupancy, which the latency-bound
+// traversal needs more than the culled pops.
+bool Engine::tn_stack_for(int block) const {
+    int want = int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), tlas_nodes));
+    size_t share = size_t(160 * 1024) * size_t(block) / 1024;
+    size_t need = size_t(want) * sizeof(oxd::BNode) + size_t(block) * stack_cap() * 8;
+    return need <= share;
+}
+
+oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int block) const {
+    const bool tn = tune.stack_mode == 1 && tn_stack_for(block);
+    const int ntop = tn ? int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), tlas_nodes)) : ntop_for(block);
     oxd::Params P{};
     P.g = geom();
     P.t = tables;
@@ -74,6 +87,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.aligned = cfg.rng_schedule == OXR_RNG_ALIGNED ? 1 : 0;
     P.ntop = ntop;
     P.stack_cap = stack_cap();
+    P.tn_stack = tn ? 1 : 0;
     P.refill_threshold = std::max(1, tune.refill_batch);
     P.segs = nullptr;
     P.nseg = 0;
@@ -145,8 +159,7 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     auto t0 = std::chrono::steady_clock::now();
     int block = form == OXR_FORM_PERSISTENT ? e.tune.block_size : 256;
     if (block < 64 || block > 1024 || block % 64) return fail(OXR_EINVAL, "tuning block_size must be a multiple of 64 in [64,1024]");
-    int ntop = e.ntop_for(block);
-    oxd::Params P = e.params(b, n, slot_off, evt_idx, ntop);
+    oxd::Params P = e.params(b, n, slot_off, evt_idx, block);
     P.pre = pre; P.npre = npre; P.input = input;
     size_t smem = oxl::simulate_smem(P, block);
     if (smem > 64 * 1024) return fail(OXR_EINVAL, "LDS budget exceeded: lower block_size or lds_top_nodes");
@@ -355,6 +368,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tune.blocks_per_cu = 3;
     e->tune.lds_top_nodes = 64;
     e->tune.refill_batch = 16;
+    e->tune.stack_mode = 1;
     auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
     if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
@@ -534,7 +548,7 @@ int oxr_engine_set_tuning(oxr_engine* eh, const oxr_tuning* t) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !t) return fail(OXR_EINVAL, "null argument");
     if (t->block_size < 64 || t->block_size > 1024 || t->block_size % 64 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
-        t->lds_top_nodes < 0 || t->refill_batch < 1)
+        t->lds_top_nodes < 0 || t->refill_batch < 1 || t->stack_mode < 0 || t->stack_mode > 1)
         return fail(OXR_EINVAL, "tuning out of range");
     std::lock_guard<std::mutex> lk(e->mu);
     e->tune = *t;
@@ -809,7 +823,7 @@ int oxr_trace_rays(oxr_engine* eh, const float* rays, int64_t n, float* prd) {
     if ((x = dp.reserve(size_t(n) * 2)) != hipSuccess) { cleanup(); return hip_fail(x, "hipMalloc prd"); }
     if ((x = hipMemcpy(dr.p, rays, size_t(n) * 32, hipMemcpyHostToDevice)) != hipSuccess) { cleanup(); return hip_fail(x, "copy rays"); }
     EventBuffers dummy;
-    oxd::Params P = e->params(dummy, 0, 0, 0, e->ntop_for(256));
+    oxd::Params P = e->params(dummy, 0, 0, 0, 256);
     if ((x = oxl::trace_rays(e->stream, P, dr.p, n, dp.p, e->cfg.fp64_roots != 0, e->has_list)) != hipSuccess) { cleanup(); return hip_fail(x, "trace launch"); }
     if ((x = hipStreamSynchronize(e->stream)) != hipSuccess) { cleanup(); return hip_fail(x, "trace sync"); }
     x = hipMemcpy(prd, dp.p, size_t(n) * 32, hipMemcpyDeviceToHost);
@@ -852,7 +866,7 @@ int oxr_simtrace(oxr_engine* eh, const float* gs, int64_t num_gs, float* simtrac
         cleanup(); return hip_fail(x, "copy FRAME gensteps");
     }
     EventBuffers dummy;
-    oxd::Params P = e->params(dummy, 0, 0, 0, e->ntop_for(256));
+    oxd::Params P = e->params(dummy, 0, 0, 0, 256);
     if ((x = oxl::simtrace(e->stream, P, dg.p, doff.p, int(num_gs), n, dout.p, e->cfg.fp64_roots != 0, e->has_list)) != hipSuccess) {
         cleanup(); return hip_fail(x, "simtrace launch");
     }
@@ -907,18 +921,20 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         }
         ms = tot / repeat;
         if (results && nres < max_results) {
-            double* r = results + 5 * nres;
-            r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch; r[4] = ms;
+            double* r = results + 6 * nres;
+            r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch;
+            r[4] = t.stack_mode; r[5] = ms;
         }
         nres++;
         if (ms < best_ms) { best_ms = ms; best = t; }
         return OXR_OK;
     };
-    // coordinate search: launch shape, then LDS staging, then refill granularity
+    // coordinate search: launch shape, then LDS staging, then refill granularity, then
+    // the traversal stack form
     for (int bs : {64, 128, 256}) {
         for (int bpc : {1, 2, 3, 4, 6, 8}) {
             oxr_tuning t = base; t.block_size = bs; t.blocks_per_cu = bpc;
-            if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, 0), bs) > 64 * 1024) continue;
+            if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, bs), bs) > 64 * 1024) continue;
             double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
         }
     }
@@ -930,6 +946,11 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
     oxr_tuning b2 = best;
     for (int rb : {1, 8, 16, 32, 48, 64}) {
         oxr_tuning t = b2; t.refill_batch = rb;
+        double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+    }
+    oxr_tuning b3 = best;
+    for (int sm : {0, 1}) {
+        oxr_tuning t = b3; t.stack_mode = sm;
         double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
     }
     oxr_engine_set_tuning(eh, &best);

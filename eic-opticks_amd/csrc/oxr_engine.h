@@ -95,9 +95,10 @@ struct Engine {
     std::mutex mu;   // serialises C-ABI calls on one engine
 
     oxd::Geom geom() const;
-    oxd::Params params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int ntop) const;
+    oxd::Params params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int block) const;
     int stack_cap() const;
     int ntop_for(int block) const;
+    bool tn_stack_for(int block) const;
 };
 
 // upload a gensteps slice into b, fill the seed array; returns photons in the slice

@@ -52,6 +52,7 @@ struct Params {
     int aligned;
     int ntop;                  // top-level BVH nodes staged in LDS
     int stack_cap;             // per-lane traversal stack entries in LDS
+    int tn_stack;              // stack entries carry their box entry distance (8 B each)
     int refill_threshold;      // persistent form: idle lanes per wave before a refill
     const Seg* segs;           // batched serving step, or null for a single event
     int nseg;

@@ -192,7 +192,7 @@ int Server::launch_step(Slot& s) {
     OXR_HIP(hipMemcpyAsync(s.d_segs.p, s.segs.data(), s.segs.size() * sizeof(oxd::Seg), hipMemcpyHostToDevice, s.stream));
     int form = cfg.form;
     int block = form == OXR_FORM_PERSISTENT ? e->tune.block_size : 256;
-    oxd::Params P = e->params(s.buf, s.N, 0, 0, e->ntop_for(block));
+    oxd::Params P = e->params(s.buf, s.N, 0, 0, block);
     P.seq = nullptr; P.record = nullptr;
     P.segs = s.d_segs.p; P.nseg = int(s.segs.size());
     OXR_HIP(hipMemsetAsync(s.buf.counters.p, 0, 8 * sizeof(unsigned long long), s.stream));

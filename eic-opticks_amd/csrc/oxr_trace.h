@@ -73,11 +73,15 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 }
 
 // Closest hit in (tmin, tmax]; ties in t go to the smaller (instance, local prim) key.
-// stk points at this lane's LDS stack column (element k at stk[k*stride]).
+// stk points at this lane's LDS stack column (element k at stk[k*stride]). With TN,
+// a second column of cap floats follows the stack (at stk + stride*cap) holding each
+// pushed node's box entry distance, and a popped node whose entry lies beyond the best
+// hit so far is dropped without loading it: the same cull the box test applies, so the
+// hit found is the same.
 // During traversal only (t, instance, local prim, object-frame normal) of the best hit
 // stay live; its world normal, local position and identity words are formed afterwards
 // from the instance record, with the same arithmetic as at instance entry.
-template <bool FP64, bool COUNT, bool LIST>
+template <bool FP64, bool COUNT, bool LIST, bool TN = false>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
@@ -89,6 +93,15 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
     int cur = g.tlas_root;
     bool at_top = true;
     f3 o = pos, d = mom, inv = safe_inv(mom), noi = neg_oi(pos, inv);
+    float* stkt = reinterpret_cast<float*>(stk + stride * cap);
+    // pop the next entry worth visiting; false when the stack is empty
+    auto pop = [&]() -> bool {
+        while (sp > 0) {
+            sp--;
+            if (!TN || stkt[sp * stride] <= best) { cur = stk[sp * stride]; return true; }
+        }
+        return false;
+    };
     int inst = 0;
     int primOffset = 0;
     for (;;) {
@@ -104,23 +117,27 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                 bool first0 = tn0 <= tn1;
                 int nearc = first0 ? nd.ch.x : nd.ch.y;
                 int farc = first0 ? nd.ch.y : nd.ch.x;
-                if (sp < cap) { stk[sp * stride] = farc; sp++; } else { err |= 1u; }
+                if (sp < cap) {
+                    stk[sp * stride] = farc;
+                    if (TN) stkt[sp * stride] = first0 ? tn1 : tn0;
+                    sp++;
+                } else {
+                    err |= 1u;
+                }
                 cur = nearc;
             } else if (h0) {
                 cur = nd.ch.x;
             } else if (h1) {
                 cur = nd.ch.y;
-            } else {
-                if (sp == 0) break;
-                sp--; cur = stk[sp * stride];
+            } else if (!pop()) {
+                break;
             }
             continue;
         }
         if (cur == SENTINEL) {
             at_top = true;
             o = pos; d = mom; inv = safe_inv(mom); noi = neg_oi(pos, inv);
-            if (sp == 0) break;
-            sp--; cur = stk[sp * stride];
+            if (!pop()) break;
             continue;
         }
         int item = ~cur;
@@ -134,7 +151,13 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                 inv = safe_inv(d);
                 noi = neg_oi(o, inv);
             }
-            if (sp < cap) { stk[sp * stride] = SENTINEL; sp++; } else { err |= 1u; }
+            if (sp < cap) {
+                stk[sp * stride] = SENTINEL;
+                if (TN) stkt[sp * stride] = -INFINITY;
+                sp++;
+            } else {
+                err |= 1u;
+            }
             at_top = false;
             cur = in.info.w;
             continue;
@@ -161,8 +184,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                 bn = mk(is.x, is.y, is.z);
             }
         }
-        if (sp == 0) break;
-        sp--; cur = stk[sp * stride];
+        if (!pop()) break;
     }
     if (!found) {
         hit.t = 0.f; hit.n = mk(0.f, 0.f, 0.f); hit.lpos = mk(0.f, 0.f, 0.f);

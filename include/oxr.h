@@ -138,6 +138,10 @@ typedef struct oxr_tuning {
     int32_t blocks_per_cu;    /* persistent grid = blocks_per_cu * num_CU              */
     int32_t lds_top_nodes;    /* top-level BVH nodes staged in LDS per workgroup       */
     int32_t refill_batch;     /* photons claimed per wave per queue visit (x64 lanes)  */
+    int32_t stack_mode;       /* 1: traversal stack entries carry their box entry
+                                 distance and popped nodes beyond the best hit are
+                                 skipped (production kernels, when the block's LDS
+                                 share at 4 waves/SIMD allows); 0: node indices only */
 } oxr_tuning;
 
 /* ---- outputs ---------------------------------------------------------- */
@@ -287,8 +291,8 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
 
 /* ---- autotuner ----------------------------------------------------------
  * Times the persistent form on the current event over a grid of oxr_tuning values
- * (repeat times each) and applies the fastest. results (max_results,5) f64 =
- * block_size, blocks_per_cu, lds_top_nodes, refill_batch, kernel_ms. */
+ * (repeat times each) and applies the fastest. results (max_results,6) f64 =
+ * block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, kernel_ms. */
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 
