@@ -55,6 +55,8 @@ struct Rng {
     uint64_t pos;             // absolute uniform index of the next uniform
     uint32_t b0, b1, b2, b3;  // unconsumed outputs of the current block, b0 next
     int have;                 // how many of b0..b3 are valid
+    uint32_t c0, c1, c2, c3;  // the following block, computed ahead by rng_topup
+    int have2;                // 4 when c0..c3 hold it, else 0
     const float* pre;         // precooked row or null
     int32_t npre, cur;
     uint32_t overflow;
@@ -82,8 +84,9 @@ OXD void rng_init(Rng& r, uint64_t seed, uint64_t offset, uint64_t skip, uint64_
     r.k0 = uint32_t(seed); r.k1 = uint32_t(seed >> 32);
     r.p0 = uint32_t(photon); r.p1 = uint32_t(photon >> 32);
     r.pos = offset + skip * event;
-    r.have = 0;
+    r.have = 0; r.have2 = 0;
     r.b0 = r.b1 = r.b2 = r.b3 = 0u;
+    r.c0 = r.c1 = r.c2 = r.c3 = 0u;
     r.pre = nullptr; r.npre = 0; r.cur = 0; r.overflow = 0;
 }
 
@@ -92,9 +95,39 @@ OXD void rng_resume(Rng& r, uint64_t seed, uint64_t pos, uint64_t photon) {
     r.k0 = uint32_t(seed); r.k1 = uint32_t(seed >> 32);
     r.p0 = uint32_t(photon); r.p1 = uint32_t(photon >> 32);
     r.pos = pos;
-    r.have = 0;
+    r.have = 0; r.have2 = 0;
     r.b0 = r.b1 = r.b2 = r.b3 = 0u;
+    r.c0 = r.c1 = r.c2 = r.c3 = 0u;
     r.pre = nullptr; r.npre = 0; r.cur = 0; r.overflow = 0;
+}
+
+// refill an empty buffer: from the block computed ahead, else from Philox
+OXD void rng_refill(Rng& r) {
+    if (r.have2) {                             // the block after a buffer always starts aligned
+        r.b0 = r.c0; r.b1 = r.c1; r.b2 = r.c2; r.b3 = r.c3;
+        r.have = 4; r.have2 = 0;
+        return;
+    }
+    uint64_t block = r.pos >> 2;
+    philox(uint32_t(block), uint32_t(block >> 32), r.p0, r.p1, r.k0, r.k1, r.b0, r.b1, r.b2, r.b3);
+    uint32_t skip = uint32_t(r.pos & 3);      // only the first block of a stream can start mid-block
+    if (skip >= 1u) { r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3; }
+    if (skip >= 2u) { r.b0 = r.b1; r.b1 = r.b2; }
+    if (skip >= 3u) { r.b0 = r.b1; }
+    r.have = 4 - int(skip);
+}
+
+// Compute the block after the buffered uniforms now, so that the next five to eight
+// uniforms need no Philox. Called where every lane of a branch is about to draw (the
+// start of a bounce's physics, the diffuse-reflection rejection loops): the Philox rounds
+// then run once for the branch instead of inside divergent loops whenever some lane's
+// buffer empties. The values drawn are unchanged; only when they are computed moves.
+OXD void rng_topup(Rng& r) {
+    if (r.pre || r.have2) return;
+    if (r.have == 0) rng_refill(r);
+    uint64_t next = (r.pos + uint64_t(r.have)) >> 2;
+    philox(uint32_t(next), uint32_t(next >> 32), r.p0, r.p1, r.k0, r.k1, r.c0, r.c1, r.c2, r.c3);
+    r.have2 = 4;
 }
 
 OXD float uniform(Rng& r) {
@@ -102,15 +135,7 @@ OXD float uniform(Rng& r) {
         if (r.cur >= r.npre) { r.cur = 0; r.overflow++; }
         return r.pre[r.cur++];
     }
-    if (r.have == 0) {
-        uint64_t block = r.pos >> 2;
-        philox(uint32_t(block), uint32_t(block >> 32), r.p0, r.p1, r.k0, r.k1, r.b0, r.b1, r.b2, r.b3);
-        uint32_t skip = uint32_t(r.pos & 3);      // only the first block of a stream can start mid-block
-        if (skip >= 1u) { r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3; }
-        if (skip >= 2u) { r.b0 = r.b1; r.b1 = r.b2; }
-        if (skip >= 3u) { r.b0 = r.b1; }
-        r.have = 4 - int(skip);
-    }
+    if (r.have == 0) rng_refill(r);
     uint32_t x = r.b0;
     r.b0 = r.b1; r.b1 = r.b2; r.b2 = r.b3;
     r.have--;

@@ -241,20 +241,36 @@ OXD f3 random_direction_marsaglia(Rng& r) {
     a = 2.f * sqrtf(1.f - b);
     return mk(a * u, a * v, 2.f * b - 1.f);
 }
-// qsim::lambertian_direction (qudarap/qsim.h:430-483)
+// qsim::lambertian_direction (qudarap/qsim.h:430-483) with random_direction_marsaglia
+// (qsim.h:551-572) inlined. The reference nests the Marsaglia rejection loop inside the
+// Lambertian acceptance loop; here both run as one loop, one Marsaglia try per turn, so
+// the lanes of a wave advance through their own tries independently instead of every
+// outer turn waiting for the slowest lane's inner loop. Each lane draws the same uniforms
+// in the same order, and both turn caps (MAX_REJECT Marsaglia tries per direction, 1024
+// directions) end the loop where the nested form ends.
 OXD f3 lambertian_direction(f3 normal, float orient, Rng& r) {
-    float ndotv; int count = 0; float u; f3 dir;
-    do {
+    int count = 0, turns = 0;
+    f3 dir = mk(0.f, 0.f, 0.f);
+    for (;;) {
+        float u0 = uniform(r), u1 = uniform(r);
+        float mu = 2.f * u0 - 1.f;
+        float mv = 2.f * u1 - 1.f;
+        float b = mu * mu + mv * mv;
+        if (b > 1.f && ++turns < MAX_REJECT) continue;        // Marsaglia: try again
+        turns = 0;
         count++;
-        dir = random_direction_marsaglia(r);
-        ndotv = dot(dir, normal) * orient;
+        float a = 2.f * sqrtf(1.f - b);
+        dir = mk(a * mu, a * mv, 2.f * b - 1.f);
+        float ndotv = dot(dir, normal) * orient;
         if (ndotv < 0.f) { dir = mul(-1.f, dir); ndotv = -1.f * ndotv; }
-        u = uniform(r);
-    } while (!(u < ndotv) && (count < 1024));
+        float u = uniform(r);
+        if (u < ndotv || count >= 1024) break;
+    }
     return dir;
 }
 // qsim::reflect_diffuse (qudarap/qsim.h:1977-2012)
 OXD void reflect_diffuse(Ph& p, f3 normal, Rng& r) {
+    rng_topup(r);
     f3 old_mom = p.mom;
     float orient = dot(old_mom, normal) > 0.f ? -1.f : 1.f;
     p.mom = lambertian_direction(normal, orient, r);
@@ -294,6 +310,7 @@ OXD int propagate(Ph& p, const Hit& h, f3 normal, float lposcost, Rng& r, const 
     set_prd(p, boundary, identity, cosTheta, iindex);
     St s;
     fill_state(t, s, boundary, p.wavelength, cosTheta);
+    rng_topup(r);
     unsigned flag = 0;
     int command = propagate_to_boundary(flag, p, s, h.t, r, t, aligned);
     if (command == BOUNDARY) {
