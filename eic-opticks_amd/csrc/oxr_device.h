@@ -25,7 +25,32 @@ OXD f3 cross(f3 a, f3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.
 OXD float len(f3 a) { return sqrtf(dot(a, a)); }
 OXD f3 normalize(f3 a) { float inv = 1.0f / sqrtf(dot(a, a)); return mulr(a, inv); }
 
-OXD float xlog(float x) { return (float)log((double)x); }
+// Natural log in single precision for the photon loop: the argument reduction and
+// minimax polynomial of FreeBSD msun's e_logf.c (x = 2^k (1+f), sqrt(2)/2 <= 1+f < sqrt(2),
+// s = f/(2+f), log(1+f) = f - f^2/2 + s (f^2/2 + R(s^2))), in IEEE single-precision
+// operations only, so the oracle's restatement gives the same bits. Faithfully rounded
+// (max 0.83 ulp over the 2^24 Philox uniforms). Zero, negative, subnormal, infinite
+// and NaN arguments take the double-precision routine.
+OXD float xlog(float x) {
+    uint32_t ix = __float_as_uint(x);
+    if (ix >= 0x7f800000u || ix < 0x00800000u) return (float)log((double)x);
+    const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f;
+    const float Lg1 = 0.66666662693f, Lg2 = 0.40000972152f, Lg3 = 0.28498786688f, Lg4 = 0.24279078841f;
+    int k = int(ix >> 23) - 127;
+    uint32_t m = ix & 0x007fffffu;
+    uint32_t i = (m + 0x004afb20u) & 0x00800000u;          // set when 1.m >= sqrt(2): use (1.m)/2
+    float xm = __uint_as_float(m | (i ^ 0x3f800000u));
+    k += int(i >> 23);
+    float f = xm - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s, w = z * z;
+    float t1 = w * (Lg2 + w * Lg4);
+    float t2 = z * (Lg1 + w * Lg3);
+    float R = t2 + t1;
+    float hfsq = 0.5f * f * f;
+    float dk = float(k);
+    return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
 OXD float xsin(float x) { return (float)sin((double)x); }
 OXD float xcos(float x) { return (float)cos((double)x); }
 OXD float xatan2(float y, float x) { return (float)atan2((double)y, (double)x); }

@@ -42,6 +42,29 @@ template <typename T> inline V3<T> normalize(V3<T> a) { T inv = T(1) / std::sqrt
 
 // transcendental functions: double evaluation rounded to T
 template <typename T> inline T xlog(T x) { return T(std::log(double(x))); }
+// single precision: the device's log (FreeBSD msun e_logf.c reduction and polynomial in
+// IEEE single operations), restated so both give the same bits
+template <> inline float xlog<float>(float x) {
+    uint32_t ix; std::memcpy(&ix, &x, 4);
+    if (ix >= 0x7f800000u || ix < 0x00800000u) return float(std::log(double(x)));
+    const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f;
+    const float Lg1 = 0.66666662693f, Lg2 = 0.40000972152f, Lg3 = 0.28498786688f, Lg4 = 0.24279078841f;
+    int k = int(ix >> 23) - 127;
+    uint32_t m = ix & 0x007fffffu;
+    uint32_t i = (m + 0x004afb20u) & 0x00800000u;
+    uint32_t xb = m | (i ^ 0x3f800000u);
+    float xm; std::memcpy(&xm, &xb, 4);
+    k += int(i >> 23);
+    float f = xm - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s, w = z * z;
+    float t1 = w * (Lg2 + w * Lg4);
+    float t2 = z * (Lg1 + w * Lg3);
+    float R = t2 + t1;
+    float hfsq = 0.5f * f * f;
+    float dk = float(k);
+    return dk * ln2_hi - ((hfsq - (s * (hfsq + R) + dk * ln2_lo)) - f);
+}
 template <typename T> inline T xsin(T x) { return T(std::sin(double(x))); }
 template <typename T> inline T xcos(T x) { return T(std::cos(double(x))); }
 template <typename T> inline T xatan2(T y, T x) { return T(std::atan2(double(y), double(x))); }
@@ -1690,6 +1713,11 @@ void orc_trace_rays(void* cv, const float* rays, int64_t n, float* prd, int32_t 
             o[4] = q.lposcost; o[5] = q.lposfphi; o[6] = orc::u2f(q.iindex_identity); o[7] = orc::u2f(q.gp_boundary);
         }
     }
+}
+
+// single-precision log of the photon loop, for its accuracy pin
+void orc_xlog(const float* x, int64_t n, float* out) {
+    for (int64_t i = 0; i < n; i++) out[i] = orc::xlog<float>(x[i]);
 }
 
 // simtrace over FRAME gensteps: qsim::generate_photon_simtrace_frame (qudarap/qsim.h:2459-2510)

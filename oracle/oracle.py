@@ -43,6 +43,7 @@ def lib():
         L.orc_intersect_prim.argtypes = [vp, i32, vp, i64, vp, vp, i32]
         L.orc_trace_rays.argtypes = [vp, vp, i64, vp, i32]
         L.orc_simtrace.argtypes = [vp, vp, i64, vp, i64]
+        L.orc_xlog.argtypes = [vp, i64, vp]
         L.orc_simtrace.restype = i64
         L.orc_simulate.restype = i64
         L.orc_simulate.argtypes = [vp, vp, i64, i32, u64, vp, i32, vp, i32, i32, i64, i64,
@@ -239,3 +240,11 @@ class Oracle:
         out = np.zeros(len(u), dtype=np.float32)
         lib().orc_scint_wavelength(self.h, P(u), len(u), P(out))
         return out
+
+
+def xlog(x):
+    """The photon loop's single-precision log, elementwise."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty_like(x)
+    lib().orc_xlog(P(x), x.size, P(out))
+    return out
