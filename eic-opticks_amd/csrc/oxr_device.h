@@ -681,8 +681,8 @@ OXD bool intersect_node_discontiguous(Is& is, const float4* node, int ni, int ro
                                       const float4* itra, float t_min, f3 ro, f3 rd) {
     const unsigned num_sub = __float_as_uint(node[4 * ni].x), offset_sub = __float_as_uint(node[4 * ni].y);
     Is closest; closest.x = 0.f; closest.y = 0.f; closest.z = 0.f; closest.w = RT_DEFAULT_MAX;
+    Is s; s.x = 0.f; s.y = 0.f; s.z = 0.f; s.w = 0.f;   // one per call: a leaf reporting valid without writing keeps the last values
     for (unsigned isub = 0; isub < num_sub; isub++) {
-        Is s;
         if (intersect_leaf<FP64>(s, node, root + int(offset_sub + isub), plan, itra, t_min, ro, rd))
             if (s.w < closest.w) closest = s;
     }
@@ -704,8 +704,8 @@ OXD bool intersect_node_contiguous(Is& is, const float4* node, int ni, int root,
     Is farthest_exit; farthest_exit.x = 0.f; farthest_exit.y = 0.f; farthest_exit.z = 0.f; farthest_exit.w = t_min;
     int exit_count = 0;
     const float propagate_epsilon = 0.0001f;
+    Is s; s.x = 0.f; s.y = 0.f; s.z = 0.f; s.w = 0.f;   // shared by all three passes
     for (int i = 0; i < num_sub; i++) {
-        Is s;
         if (intersect_leaf<FP64>(s, node, root + offset_sub + i, plan, itra, t_min, ro, rd)) {
             int st = classify(s, rd, t_min);
             if (st == 0) { if (s.w < nearest_enter.w) nearest_enter = s; }
@@ -721,7 +721,6 @@ OXD bool intersect_node_contiguous(Is& is, const float4* node, int ni, int root,
     float enter[CONTIGUOUS_MAX_SUB];
     int aux[CONTIGUOUS_MAX_SUB], idx[CONTIGUOUS_MAX_SUB];
     for (int isub = 0; isub < num_sub; isub++) {
-        Is s;
         if (intersect_leaf<FP64>(s, node, root + offset_sub + isub, plan, itra, t_min, ro, rd)) {
             int st = classify(s, rd, t_min);
             if (st == 0) {
@@ -743,7 +742,6 @@ OXD bool intersect_node_contiguous(Is& is, const float4* node, int ni, int root,
         float tminAdvanced = enter[idx[i]] + propagate_epsilon;
         int isub = aux[idx[i]];
         if (tminAdvanced < farthest_exit.w) {
-            Is s;
             if (intersect_leaf<FP64>(s, node, root + offset_sub + isub, plan, itra, tminAdvanced, ro, rd)) {
                 if (classify(s, rd, tminAdvanced) == 1) {
                     exit_count += 1;
@@ -767,20 +765,19 @@ OXD bool intersect_node_overlap(Is& is, const float4* node, int ni, int root, co
     Is nearest_exit; nearest_exit.x = 0.f; nearest_exit.y = 0.f; nearest_exit.z = 0.f; nearest_exit.w = RT_DEFAULT_MAX;
     const float propagate_epsilon = 0.0001f;
     unsigned enter_count = 0, exit_count = 0;
+    Is s; s.x = 0.f; s.y = 0.f; s.z = 0.f; s.w = 0.f;   // one per call, reused for the exit search
     for (unsigned isub = 0; isub < num_sub; isub++) {
         const int sn = root + int(offset_sub + isub);
-        Is s;
         if (intersect_leaf<FP64>(s, node, sn, plan, itra, t_min, ro, rd)) {
             int st = classify(s, rd, t_min);
             if (st == 0) {
                 enter_count += 1;
                 if (s.w > farthest_enter.w) farthest_enter = s;
                 float tminAdvanced = s.w + propagate_epsilon;
-                Is s2;
-                if (intersect_leaf<FP64>(s2, node, sn, plan, itra, tminAdvanced, ro, rd)) {
-                    if (classify(s2, rd, tminAdvanced) == 1) {
+                if (intersect_leaf<FP64>(s, node, sn, plan, itra, tminAdvanced, ro, rd)) {
+                    if (classify(s, rd, tminAdvanced) == 1) {
                         exit_count += 1;
-                        if (s2.w < nearest_exit.w) nearest_exit = s2;
+                        if (s.w < nearest_exit.w) nearest_exit = s;
                     }
                 }
             } else if (st == 1) {
@@ -839,7 +836,7 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
                 continue;
             }
             if (tc >= CSG_NODE) {
-                Is ni_is;
+                Is ni_is; ni_is.x = 0.f; ni_is.y = 0.f; ni_is.z = 0.f; ni_is.w = 0.f;
                 intersect_node<FP64, LIST>(ni_is, node, ni, root, plan, itra, tmin, ro, rd);
                 ni_is.w = copysignf(ni_is.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
                 if (sp >= 14) { ierr = 1; break; }
@@ -901,6 +898,7 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
 template <bool FP64, bool LIST>
 OXD bool intersect_prim(Is& is, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
+    is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;   // zeroed isect of CSGOptiX7.cu:894-898
     unsigned tc = __float_as_uint(node[4 * root + 3].z);
     if (tc >= CSG_LEAF) return intersect_leaf<FP64>(is, node, root, plan, itra, t_min, ro, rd);
     if (tc < CSG_NODE) return intersect_tree<FP64, LIST>(is, node, root, plan, itra, t_min, ro, rd);

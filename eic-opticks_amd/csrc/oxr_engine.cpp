@@ -32,16 +32,18 @@ static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); retu
 oxd::Geom Engine::geom() const {
     oxd::Geom g;
     g.node = node.p; g.plan = plan.p; g.itra = itra.p; g.prim = prim.p;
-    g.tlas = tlas.p; g.blas = blas.p; g.inst = inst.p;
-    g.tlas_root = tlas_root; g.num_inst = num_inst;
+    const bool f = use_flat();
+    g.tlas = f ? tlas_f.p : tlas.p; g.blas = blas.p; g.inst = inst.p;
+    g.flat = f ? flat.p : nullptr;
+    g.tlas_root = f ? tlas_root_f : tlas_root; g.num_inst = num_inst;
     return g;
 }
 
-int Engine::stack_cap() const { return std::max(4, stack_need + 2); }
+int Engine::stack_cap() const { return std::max(4, (use_flat() ? stack_need_f : stack_need) + 2); }
 
 int Engine::ntop_for(int block) const {
     int want = std::max(0, tune.lds_top_nodes);
-    want = int(std::min<int64_t>(want, tlas_nodes));
+    want = int(std::min<int64_t>(want, top_nodes()));
     size_t lim = 64 * 1024;
     size_t stack = size_t(block) * stack_cap() * sizeof(int);
     if (stack >= lim) return 0;
@@ -54,7 +56,7 @@ int Engine::ntop_for(int block) const {
 // lanes per CU): more LDS per block would cost occupancy, which the latency-bound
 // traversal needs more than the culled pops.
 bool Engine::tn_stack_for(int block) const {
-    int want = int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), tlas_nodes));
+    int want = int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), top_nodes()));
     size_t share = size_t(160 * 1024) * size_t(block) / 1024;
     size_t need = size_t(want) * sizeof(oxd::BNode) + size_t(block) * stack_cap() * 8;
     return need <= share;
@@ -62,7 +64,7 @@ bool Engine::tn_stack_for(int block) const {
 
 oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int block) const {
     const bool tn = tune.stack_mode == 1 && tn_stack_for(block);
-    const int ntop = tn ? int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), tlas_nodes)) : ntop_for(block);
+    const int ntop = tn ? int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), top_nodes())) : ntop_for(block);
     oxd::Params P{};
     P.g = geom();
     P.t = tables;
@@ -369,6 +371,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tune.lds_top_nodes = 64;
     e->tune.refill_batch = 16;
     e->tune.stack_mode = 1;
+    e->tune.bvh_mode = 0;
     auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
     if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
@@ -476,11 +479,53 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
         }
         inst_box[size_t(i)] = wb;
     }
-    oxh::Bvh tl = oxh::build_bvh(inst_box, 0, slack_world);
+    // Flattened form, built besides the two levels while the placed prims number at most
+    // kFlatMax and used when the tuning asks (bvh_mode 1): one BVH over every (instance,
+    // prim) with world boxes. A trace then skips the BLAS roots and the return to the top
+    // level, and prims are culled by their own world boxes rather than their instance's;
+    // scenes that mix huge and small boxes (concentric shells around small instances) can
+    // trace slower on it, so the autotuner picks. Each prim is still tested in its
+    // instance's frame with the same arithmetic and the closest-hit key is the same, so the
+    // hits are identical.
+    const int64_t kFlatMax = int64_t(1) << 21;
+    int64_t placed = 0;
+    for (int64_t i = 0; i < g->num_inst; i++) {
+        int32_t gas; std::memcpy(&gas, g->inst + 16 * i + 7, 4);
+        placed += e->solid_num_prim[size_t(gas)];
+    }
+    e->has_flat = placed <= kFlatMax;
+    oxh::Bvh tl, tf;
+    std::vector<int2> flat_items;
+    if (e->has_flat) {
+        std::vector<oxh::Box> leaf_box;
+        leaf_box.reserve(size_t(placed)); flat_items.reserve(size_t(placed));
+        for (int64_t i = 0; i < g->num_inst; i++) {
+            const float* q = g->inst + 16 * i;
+            int32_t gas; std::memcpy(&gas, q + 7, 4);
+            const std::vector<oxh::Box>& pb = prim_boxes[size_t(gas)];
+            for (size_t p = 0; p < pb.size(); p++) {
+                oxh::Box wb = oxh::Box::empty();
+                for (int c = 0; c < 8; c++) {
+                    double pt[3] = {(c & 1) ? pb[p].hi[0] : pb[p].lo[0], (c & 2) ? pb[p].hi[1] : pb[p].lo[1],
+                                    (c & 4) ? pb[p].hi[2] : pb[p].lo[2]};
+                    double w[3];
+                    for (int k = 0; k < 3; k++) w[k] = pt[0] * q[k] + pt[1] * q[4 + k] + pt[2] * q[8 + k] + q[12 + k];
+                    wb.grow_point(w);
+                }
+                leaf_box.push_back(wb);
+                flat_items.push_back(make_int2(int(i), int(p)));
+            }
+        }
+        tf = oxh::build_bvh(leaf_box, 0, slack_world);
+        e->stack_need_f = tf.depth + 1;
+        e->tlas_root_f = tf.root_ref;
+        e->tlas_nodes_f = int64_t(tf.nodes.size());
+    }
+    tl = oxh::build_bvh(inst_box, 0, slack_world);
+    e->stack_need = tl.depth + 1 + max_blas_depth;
     e->tlas_root = tl.root_ref;
     e->tlas_nodes = int64_t(tl.nodes.size());
     e->blas_nodes = int64_t(blas_all.size());
-    e->stack_need = tl.depth + 1 + max_blas_depth;
     if (e->stack_cap() > 96) return bail(fail(OXR_EINVAL, "BVH too deep for the traversal stack"));
 
     // ---- uploads
@@ -501,6 +546,8 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     if ((rc = up(e->prim, primtab.data(), primtab.size(), 8))) return bail(rc);
     if ((rc = up(e->tlas, tl.nodes.data(), tl.nodes.size(), 64))) return bail(rc);
     if ((rc = up(e->blas, blas_all.data(), blas_all.size(), 64))) return bail(rc);
+    if ((rc = up(e->flat, flat_items.data(), flat_items.size(), 8))) return bail(rc);
+    if ((rc = up(e->tlas_f, tf.nodes.data(), tf.nodes.size(), 64))) return bail(rc);
     if ((rc = up(e->inst, insts.data(), insts.size(), 64))) return bail(rc);
     size_t bnd_f4 = size_t(ph->num_bnd) * 4 * 2 * size_t(ph->num_wl);
     if ((rc = up(e->bnd, ph->bnd, bnd_f4, 16))) return bail(rc);
@@ -525,7 +572,7 @@ void oxr_engine_destroy(oxr_engine* eh) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     e->ev.release_all();
     e->node.release(); e->plan.release(); e->itra.release(); e->bnd.release(); e->prim.release();
-    e->tlas.release(); e->blas.release(); e->inst.release(); e->optical.release(); e->icdf.release();
+    e->tlas.release(); e->blas.release(); e->inst.release(); e->flat.release(); e->tlas_f.release(); e->optical.release(); e->icdf.release();
     if (e->e0) (void)hipEventDestroy(e->e0);
     if (e->e1) (void)hipEventDestroy(e->e1);
     if (e->e2) (void)hipEventDestroy(e->e2);
@@ -548,7 +595,8 @@ int oxr_engine_set_tuning(oxr_engine* eh, const oxr_tuning* t) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !t) return fail(OXR_EINVAL, "null argument");
     if (t->block_size < 64 || t->block_size > 1024 || t->block_size % 64 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
-        t->lds_top_nodes < 0 || t->refill_batch < 1 || t->stack_mode < 0 || t->stack_mode > 1)
+        t->lds_top_nodes < 0 || t->refill_batch < 1 || t->stack_mode < 0 || t->stack_mode > 1 ||
+        t->bvh_mode < 0 || t->bvh_mode > 1)
         return fail(OXR_EINVAL, "tuning out of range");
     std::lock_guard<std::mutex> lk(e->mu);
     e->tune = *t;
@@ -921,19 +969,26 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         }
         ms = tot / repeat;
         if (results && nres < max_results) {
-            double* r = results + 6 * nres;
+            double* r = results + 7 * nres;
             r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch;
-            r[4] = t.stack_mode; r[5] = ms;
+            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = ms;
         }
         nres++;
         if (ms < best_ms) { best_ms = ms; best = t; }
         return OXR_OK;
     };
-    // coordinate search: launch shape, then LDS staging, then refill granularity, then
-    // the traversal stack form
+    // coordinate search: BVH form, launch shape, LDS staging, refill granularity, then the
+    // traversal stack form
+    if (e->has_flat) {
+        for (int bm : {0, 1}) {
+            oxr_tuning t = base; t.bvh_mode = bm;
+            double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+        }
+    }
+    const oxr_tuning b0 = best;
     for (int bs : {64, 128, 256}) {
         for (int bpc : {1, 2, 3, 4, 6, 8}) {
-            oxr_tuning t = base; t.block_size = bs; t.blocks_per_cu = bpc;
+            oxr_tuning t = b0; t.block_size = bs; t.blocks_per_cu = bpc;
             if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, bs), bs) > 64 * 1024) continue;
             double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
         }

@@ -68,6 +68,14 @@ struct Engine {
     DBuf<int2> prim;
     DBuf<oxd::BNode> tlas, blas;
     DBuf<oxd::InstD> inst;
+    DBuf<int2> flat;             // (instance, local prim) per leaf of the flattened BVH
+    DBuf<oxd::BNode> tlas_f;     // the flattened BVH (built when the placed prims are few enough)
+    bool has_flat = false;
+    int tlas_root_f = 0;
+    int64_t tlas_nodes_f = 0;
+    int stack_need_f = 0;
+    bool use_flat() const { return has_flat && tune.bvh_mode == 1; }
+    int64_t top_nodes() const { return use_flat() ? tlas_nodes_f : tlas_nodes; }
     DBuf<uint4> optical;
     DBuf<float> icdf;
     int tlas_root = 0;

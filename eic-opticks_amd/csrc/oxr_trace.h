@@ -10,6 +10,8 @@
 //   prim[]  int2 (nodeOffset, globalPrimIdx<<16|boundary).
 // A child ref >= 0 is an internal node, < 0 is a leaf item ~ref (one instance per TLAS
 // leaf, one prim per BLAS leaf). The traversal stack lives in LDS, one column per lane.
+//   flat[]  int2 (instance, local prim) per leaf when the geometry is flattened: one BVH
+//           over every placed prim's world box, no BLAS level (see Engine create).
 #pragma once
 #include "oxr_device.h"
 
@@ -27,6 +29,7 @@ struct Geom {
     const BNode* tlas;
     const BNode* blas;
     const InstD* inst;
+    const int2* flat;      // flattened leaves (instance, local prim), or null for two levels
     int tlas_root;
     int num_inst;
 };
@@ -141,6 +144,38 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             continue;
         }
         int item = ~cur;
+        if (g.flat) {
+            // flattened leaf: one placed prim, tested in its instance's frame; the same
+            // (instance, local prim) key and arithmetic as the two-level path
+            const int2 fi = g.flat[item];
+            const InstD in = g.inst[fi.x];
+            if (COUNT) ivis++;
+            f3 lo = pos, ld = mom;
+            if ((in.info.z & 1) == 0) to_object(in, pos, mom, lo, ld);
+            int nodeOffset = g.prim[in.info.x + fi.y].x;
+            if (COUNT) {
+                tests++;
+                unsigned tc0 = __float_as_uint(g.node[4 * nodeOffset + 3].z);
+                unsigned nn = tc0 >= CSG_LEAF ? 1u : __float_as_uint(g.node[4 * nodeOffset].x);
+                for (unsigned k = 0; k < nn; k++) {
+                    float4 q3 = g.node[4 * (nodeOffset + int(k)) + 3];
+                    csgb += 64u;
+                    if (__float_as_uint(q3.z) >= CSG_LEAF && (__float_as_uint(q3.w) & 0x7fffffffu) != 0u) csgb += 64u;
+                }
+            }
+            Is is;
+            if (intersect_prim<FP64, LIST>(is, g.node, nodeOffset, g.plan, g.itra, tmin, lo, ld)) {
+                float t = is.w;
+                bool closer = t >= tmin && (t < best || (found && t == best && (fi.x < best_inst || (fi.x == best_inst && fi.y < best_item))));
+                if (closer) {
+                    found = true;
+                    best = t; best_inst = fi.x; best_item = fi.y;
+                    bn = mk(is.x, is.y, is.z);
+                }
+            }
+            if (!pop()) break;
+            continue;
+        }
         if (at_top) {
             const InstD in = g.inst[item];
             if (COUNT) ivis++;
@@ -176,8 +211,11 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         }
         Is is;
         if (intersect_prim<FP64, LIST>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
+            // a reported distance outside the ray interval is ignored, as by
+            // optixReportIntersection (CSGOptiX/CSGOptiX7.cu:912): a convex polyhedron
+            // wholly behind the ray reports valid with the zeroed isect
             float t = is.w;
-            bool closer = t < best || (found && t == best && (inst < best_inst || (inst == best_inst && item < best_item)));
+            bool closer = t >= tmin && (t < best || (found && t == best && (inst < best_inst || (inst == best_inst && item < best_item))));
             if (closer) {
                 found = true;
                 best = t; best_inst = inst; best_item = item;

@@ -54,7 +54,8 @@ class Config(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("lds_top_nodes", C.c_int32), ("refill_batch", C.c_int32), ("stack_mode", C.c_int32)]
+                ("lds_top_nodes", C.c_int32), ("refill_batch", C.c_int32), ("stack_mode", C.c_int32),
+                ("bvh_mode", C.c_int32)]
 
 
 class LaunchStats(C.Structure):

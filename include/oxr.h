@@ -142,6 +142,9 @@ typedef struct oxr_tuning {
                                  distance and popped nodes beyond the best hit are
                                  skipped (production kernels, when the block's LDS
                                  share at 4 waves/SIMD allows); 0: node indices only */
+    int32_t bvh_mode;         /* 1: one BVH over every placed prim (world boxes), when the
+                                 engine built it (at most 2^21 placed prims); 0: instance
+                                 BVH over per-solid prim BVHs                            */
 } oxr_tuning;
 
 /* ---- outputs ---------------------------------------------------------- */
@@ -291,8 +294,8 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
 
 /* ---- autotuner ----------------------------------------------------------
  * Times the persistent form on the current event over a grid of oxr_tuning values
- * (repeat times each) and applies the fastest. results (max_results,6) f64 =
- * block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, kernel_ms. */
+ * (repeat times each) and applies the fastest. results (max_results,7) f64 =
+ * block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode, kernel_ms. */
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 

@@ -670,8 +670,8 @@ void intersect_node_discontiguous(const Ctx& c, bool& valid, Isect<T>& is, const
                                   V3<T> ro, V3<T> rd) {
     unsigned num_sub = nd->u(0), offset_sub = nd->u(1);
     Isect<T> closest{0, 0, 0, T(RT_DEFAULT_MAX)};
-    for (unsigned isub = 0; isub < num_sub; isub++) {
-        Isect<T> sub{0, 0, 0, 0};
+    Isect<T> sub{0, 0, 0, 0};                       // one per call: a leaf that reports valid
+    for (unsigned isub = 0; isub < num_sub; isub++) { // without writing leaves the last values
         bool v = false;
         intersect_leaf(c, v, sub, root[offset_sub + isub], t_min, ro, rd);
         if (v && sub.w < closest.w) closest = sub;
@@ -690,8 +690,8 @@ void intersect_node_contiguous(const Ctx& c, bool& valid, Isect<T>& is, const No
     Isect<T> farthest_exit{0, 0, 0, t_min};
     int exit_count = 0;
     const T propagate_epsilon = T(0.0001f);
+    Isect<T> sub{0, 0, 0, 0};                       // shared by all three passes
     for (int i = 0; i < num_sub; i++) {
-        Isect<T> sub{0, 0, 0, 0};
         bool v = false;
         intersect_leaf(c, v, sub, root[offset_sub + i], t_min, ro, rd);
         if (v) {
@@ -709,7 +709,6 @@ void intersect_node_contiguous(const Ctx& c, bool& valid, Isect<T>& is, const No
     T enter[8];
     int aux[8], idx[8];
     for (int isub = 0; isub < num_sub; isub++) {
-        Isect<T> sub{0, 0, 0, 0};
         bool v = false;
         intersect_leaf(c, v, sub, root[offset_sub + isub], t_min, ro, rd);
         if (v) {
@@ -733,7 +732,6 @@ void intersect_node_contiguous(const Ctx& c, bool& valid, Isect<T>& is, const No
         T tminAdvanced = enter[idx[i]] + propagate_epsilon;
         int isub = aux[idx[i]];
         if (tminAdvanced < farthest_exit.w) {
-            Isect<T> sub{0, 0, 0, 0};
             bool v = false;
             intersect_leaf(c, v, sub, root[offset_sub + isub], tminAdvanced, ro, rd);
             if (v && classify(sub, rd, tminAdvanced) == State_Exit) {
@@ -755,9 +753,9 @@ void intersect_node_overlap(const Ctx& c, bool& valid, Isect<T>& is, const Node*
     Isect<T> nearest_exit{0, 0, 0, T(RT_DEFAULT_MAX)};
     const T propagate_epsilon = T(0.0001f);
     unsigned enter_count = 0, exit_count = 0;
+    Isect<T> sub{0, 0, 0, 0};                       // one per call, reused for the exit search
     for (unsigned isub = 0; isub < num_sub; isub++) {
         const Node& sn = root[offset_sub + isub];
-        Isect<T> sub{0, 0, 0, 0};
         bool v = false;
         intersect_leaf(c, v, sub, sn, t_min, ro, rd);
         if (!v) continue;
@@ -766,12 +764,11 @@ void intersect_node_overlap(const Ctx& c, bool& valid, Isect<T>& is, const Node*
             enter_count += 1;
             if (sub.w > farthest_enter.w) farthest_enter = sub;
             T tminAdvanced = sub.w + propagate_epsilon;
-            Isect<T> s2{0, 0, 0, 0};
-            bool v2 = false;
-            intersect_leaf(c, v2, s2, sn, tminAdvanced, ro, rd);
-            if (v2 && classify(s2, rd, tminAdvanced) == State_Exit) {
+            v = false;
+            intersect_leaf(c, v, sub, sn, tminAdvanced, ro, rd);
+            if (v && classify(sub, rd, tminAdvanced) == State_Exit) {
                 exit_count += 1;
-                if (s2.w < nearest_exit.w) nearest_exit = s2;
+                if (sub.w < nearest_exit.w) nearest_exit = sub;
             }
         } else if (st == State_Exit) {
             exit_count += 1;
@@ -996,6 +993,10 @@ bool trace(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
             Isect<T> is;
             if (!intersect_prim(c, is, root, tmin, o, d)) continue;
             T t = is.w;
+            // the hit is reported with optixReportIntersection (CSGOptiX/CSGOptiX7.cu:912),
+            // which ignores a distance outside the ray interval: a convex polyhedron wholly
+            // behind the ray reports valid with the zeroed isect, t = 0 < tmin
+            if (!(t >= tmin)) continue;
             uint64_t key = (uint64_t(i) << 32) | uint64_t(p);
             if (t < best || (found && t == best && key < best_key)) {
                 found = true;

@@ -28,7 +28,7 @@ def test_struct_sizes_match_header(engine_mod):
     assert C.sizeof(abi.Geometry) == 96
     assert C.sizeof(abi.Config) == 88
     assert C.sizeof(abi.LaunchStats) == 88
-    assert C.sizeof(abi.Tuning) == 20
+    assert C.sizeof(abi.Tuning) == 24
     assert C.sizeof(abi.RequestResult) == 64
 
 

@@ -421,6 +421,46 @@ def test_oracle_culling_is_exact(oracle_lib):
     assert ((a.view(np.uint32)[:, 1, 3] & 0xffff) == 3).sum() > 100     # PMT hits present
 
 
+def test_polyhedron_behind_ray_is_a_miss(oracle_lib):
+    """A ray from outside a convex polyhedron pointing away from it cannot hit it. The leaf
+    itself reports valid (its slab interval lies behind the origin) without writing the
+    isect, as the reference's leaf does; the trace must drop that zero distance."""
+    planes = [(1, 0, 0, 50), (-1, 0, 0, 50), (0, 1, 0, 30), (0, -1, 0, 30), (0, 0, 1, 20), (0, 0, -1, 20)]
+    P = _prim_oracle(oracle_lib, convexpolyhedron(planes, (-50, -30, -20, 50, 30, 20)))
+    rng = np.random.default_rng(5)
+    o = rng.uniform(-300, 300, size=(2000, 3))
+    o = o[np.any(np.abs(o) > [50, 30, 20], axis=1)]
+    tgt = rng.uniform([-50, -30, -20], [50, 30, 20], size=(len(o), 3))
+    d = o - tgt                                          # away from a point inside
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    isect, valid = P.intersect_prim(0, _rays(o, d, 0.05), 0)
+    assert valid.all() and np.all(isect == 0.0)          # the leaf's own report
+    for prec in (0, 1):
+        prd = P.trace(_rays(o, d, 0.05), prec)
+        assert np.all(prd[:, 1, 3].view(np.uint32) == 0xffffffff) and np.all(prd[:, 0, 3] == 0.0)
+
+
+def test_list_nodes_and_polyhedron_trace_is_finite(oracle_lib):
+    """Scene of list nodes, a tetrahedron and a rotated box: the box-culled trace equals
+    the brute-force one, and every hit has a finite unit-scale normal and t >= tmin."""
+    from test_foundry_io import mixed_scene_arrays
+    g, p = mixed_scene_arrays()
+    O = oracle_lib.Oracle(g, p, abi.default_config())
+    rng = np.random.default_rng(12)
+    o = rng.uniform(-250, 250, size=(4000, 3))
+    d = rng.uniform(-100, 100, size=(4000, 3)) - o       # aimed through the objects
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    rays = _rays(o, d, 0.05)
+    a = O.trace(rays)
+    O.set_cull(True)
+    b = O.trace(rays)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    hit = a[:, 1, 3].view(np.uint32) != 0xffffffff
+    assert hit.sum() > 1000
+    n = np.linalg.norm(a[hit, 0, :3], axis=1)
+    assert np.all(np.isfinite(n)) and np.all(n > 0.5) and np.all(a[hit, 0, 3] >= np.float32(0.05))
+
+
 def test_single_precision_log(oracle_lib):
     """The photon loop's log (FreeBSD msun e_logf.c method in IEEE single operations) is
     faithfully rounded: within one ulp of the exact value over every Philox uniform
