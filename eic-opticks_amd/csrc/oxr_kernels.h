@@ -69,6 +69,22 @@ bool needs_generate(const oxd::Params& P, bool count);
 hipError_t generate(hipStream_t s, const oxd::Params& P);
 hipError_t simulate_plain(hipStream_t s, const oxd::Params& P, int block, bool fp64, bool count, bool list);
 hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int grid, bool fp64, bool list);
+// The simulate kernel variants of one (FP64 roots, list support) pair, defined in oxr_sim.h
+// and instantiated in oxr_sim_{f32,f64}[_list].hip. kind: SIM_PROD, SIM_DEBUG or SIM_COUNT;
+// tm: traversal mode, bit 0 distance-carrying stack, bit 1 flattened BVH.
+enum { SIM_PROD = 0, SIM_DEBUG = 1, SIM_COUNT = 2 };
+template <bool FP64, bool LIST>
+hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, int kind, int tm);
+template <bool FP64, bool LIST>
+hipError_t launch_simulate_persist(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool debug, int tm);
+#define OXR_SIM_EXTERN(F, L)                                                                                        \
+    extern template hipError_t launch_simulate_plain<F, L>(hipStream_t, const oxd::Params&, dim3, dim3, size_t, int, int); \
+    extern template hipError_t launch_simulate_persist<F, L>(hipStream_t, const oxd::Params&, dim3, dim3, size_t, bool, int);
+OXR_SIM_EXTERN(false, false)
+OXR_SIM_EXTERN(false, true)
+OXR_SIM_EXTERN(true, false)
+OXR_SIM_EXTERN(true, true)
+#undef OXR_SIM_EXTERN
 int compact_blocks(long long n);
 hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
                    long long* offs, float4* hit, long long hit_cap);

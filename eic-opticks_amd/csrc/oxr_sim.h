@@ -1,0 +1,310 @@
+// oxr_sim.h -- the per-photon pieces and the two simulate kernels (plain and persistent
+// form), templated on root precision (FP64), counting (COUNT), debug outputs (DBG), list-node
+// support (LIST) and the traversal mode TM (bit 0: stack entries carry box entry distances;
+// bit 1: the flattened BVH). k_generate (oxr_kernels.hip) uses the per-photon pieces; the
+// kernels are instantiated by the four oxr_sim_*.hip units, one per (FP64, LIST) pair, which
+// compile in parallel.
+#pragma once
+#include "oxr_kernels.h"
+
+
+// Minimum waves per SIMD the simulate kernels are compiled for (VGPR budget 512/N per
+// lane). The latency-bound traversal wants occupancy more than registers: measured on
+// MI355X (crystal-calorimeter event), 4 waves beat 3 and 5 for both forms.
+#ifndef OXR_PERSIST_WAVES
+#define OXR_PERSIST_WAVES 4
+#endif
+#ifndef OXR_PLAIN_WAVES
+#define OXR_PLAIN_WAVES 4
+#endif
+
+namespace oxd {
+
+// ------------------------------------------------------------------ seeding
+// genstep g with offsets[g] <= i < offsets[g+1] (offsets non-decreasing, i < offsets[num_gs])
+OXD int genstep_of(const long long* __restrict__ offsets, int num_gs, long long i) {
+    int lo = 0, hi = num_gs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (offsets[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------ per-photon pieces
+struct Lane {
+    Ph p;
+    Rng r;
+    unsigned long long sh0, sh1, sb0, sb1;   // seqhis / seqbnd
+    float lc, lf;                            // lposcost, lposfphi of the last trace (photonlite)
+    long long idx;
+    int bounce;
+};
+
+template <bool DBG>
+OXD void point(const Params& P, Lane& L) {
+    if (!DBG) return;
+    if (P.record && L.bounce < P.max_record) store_photon(P.record + 4 * (L.idx * P.max_record + L.bounce), L.p);
+    if (P.seq) {
+        unsigned slot = unsigned(L.bounce);
+        if (slot < 32) {
+            unsigned shift = 4u * (slot & 15u);
+            unsigned long long his = (unsigned long long)(unsigned(__ffs(int(ph_flag(L.p)))) & 0xfu) << shift;
+            unsigned long long bnd = (unsigned long long)(ph_boundary(L.p) & 0xfu) << shift;
+            if (slot < 16) { L.sh0 |= his; L.sb0 |= bnd; } else { L.sh1 |= his; L.sb1 |= bnd; }
+        }
+    }
+}
+
+// returns false for padding indices of a batched step
+template <bool DBG>
+OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
+    L.idx = idx;
+    const float4* gsb = P.gs;
+    unsigned long long pidx;
+    int gsi;
+    int evt = P.event_index;
+    if (P.segs) {
+        int lo = 0, hi = P.nseg - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (P.segs[mid].start <= idx) lo = mid; else hi = mid - 1;
+        }
+        const Seg sg = P.segs[lo];
+        long long li = idx - sg.start;
+        if (li >= sg.count) return false;
+        pidx = sg.photon0 + (unsigned long long)li;
+        gsb = sg.gs;
+        gsi = genstep_of(sg.off, sg.num_gs, (long long)pidx);   // served events carry no seed array
+        evt = sg.event_index;
+    } else {
+        pidx = P.slot_offset + (unsigned long long)idx;
+        gsi = P.seed[idx];
+    }
+    const float4* gs = gsb + 6 * gsi;
+    rng_init(L.r, P.rng_seed, P.rng_offset, P.skip, (unsigned long long)evt, pidx);
+    if (DBG && P.pre) { L.r.pre = P.pre + (size_t)idx * P.npre; L.r.npre = P.npre; }
+    uint32_t gencode = __float_as_uint(gs[0].x);
+    if (gencode == 6) gen_torch(L.p, gs, pidx, L.r);
+    else if (gencode == 15 || gencode == 18) gen_cerenkov(L.p, gs, L.r, P.t);
+    else if (gencode == 16 || gencode == 5) gen_scint(L.p, gs, L.r, P.t);
+    else if (gencode == 19) { load_photon(L.p, P.input + 4 * pidx); set_flag(L.p, TORCH); }
+    else { L.p = Ph{}; zero_flags(L.p); }
+    set_index(L.p, pidx);
+    L.sh0 = L.sh1 = L.sb0 = L.sb1 = 0ull;
+    if (DBG) { L.lc = 0.f; L.lf = 0.f; }
+    L.bounce = 0;
+    point<DBG>(P, L);
+    return true;
+}
+
+// one loop iteration of CSGOptiX7.cu:450-482; returns false when the photon is done
+template <bool FP64, bool COUNT, bool DBG, bool LIST, int TM>
+OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode* top, int ntop,
+                     uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
+    if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return false;
+    float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
+    Hit h;
+    bool found = trace<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0>(P.g, L.p.pos, L.p.mom, tmin, P.tmax, h, stk, stride, P.stack_cap, top, ntop,
+                                    visits, tests, err, csgb, ivis);
+    if (!found) {
+        if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0
+        return false;
+    }
+    f3 normal = normalize(h.n);
+    float lposcost = h.lpos.z / sqrtf(dot(h.lpos, h.lpos));
+    if (DBG && P.lite) {
+        L.lc = lposcost;
+        L.lf = (xatan2(h.lpos.y, h.lpos.x) + PIf) / (2.0f * PIf);   // normalize_fphi, sysrap/scuda.h:653
+    }
+    int command = propagate(L.p, h, normal, lposcost, L.r, P.t, P.aligned);
+    L.bounce++;
+    point<DBG>(P, L);
+    return command != BREAK;
+}
+
+// production launches: pick up a photon that k_generate wrote (false for padding)
+OXD bool resume_photon(const Params& P, Lane& L, long long idx) {
+    const ulonglong2 st = P.rstate[idx];
+    if (st.x == ~0ull) return false;
+    L.idx = idx;
+    load_photon(L.p, P.photon + 4 * idx);
+    rng_resume(L.r, P.rng_seed, st.x, st.y);
+    L.bounce = 0;
+    return true;
+}
+
+template <bool DBG>
+OXD bool start_photon(const Params& P, Lane& L, long long idx) {
+    if (DBG) return begin_photon<true>(P, L, idx);
+    return resume_photon(P, L, idx);
+}
+
+template <bool DBG>
+OXD void end_photon(const Params& P, const Lane& L) {
+    if (DBG && P.seq) {
+        ulonglong2* s = P.seq + 2 * L.idx;
+        s[0] = make_ulonglong2(L.sh0, L.sh1);
+        s[1] = make_ulonglong2(L.sb0, L.sb1);
+    }
+    store_photon(P.photon + 4 * L.idx, L.p);
+    // all bits of the hit mask must be set (sphoton_selector, sysrap/sphoton.h:1219-1225)
+    if (P.hitflag) P.hitflag[L.idx] = (L.p.flagmask & P.hitmask) == P.hitmask ? 1 : 0;
+    if (DBG && P.lite) P.lite[L.idx] = lite_pack(L.p, L.lc, L.lf);
+    if (DBG && P.bounce_out) P.bounce_out[L.idx] = uint32_t(L.bounce);
+    if (DBG && P.overflow_out) P.overflow_out[L.idx] = L.r.overflow;
+}
+
+OXD void stage_top(const Params& P, BNode* top, int ntop) {
+    for (int i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = P.g.tlas[i];
+}
+
+OXD unsigned long long wave_sum(unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+OXD void flush_counters(const Params& P, unsigned long long steps, unsigned long long visits,
+                        unsigned long long tests, uint32_t err, bool count,
+                        unsigned long long csgb = 0, unsigned long long ivis = 0) {
+    steps = wave_sum(steps);
+    if (count) { visits = wave_sum(visits); tests = wave_sum(tests); csgb = wave_sum(csgb); ivis = wave_sum(ivis); }
+    uint64_t anyerr = __ballot(err != 0u);
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(P.counters + 0, steps);
+        if (count) {
+            atomicAdd(P.counters + 2, visits); atomicAdd(P.counters + 3, tests);
+            atomicAdd(P.counters + 5, csgb); atomicAdd(P.counters + 6, ivis);
+        }
+        if (anyerr) atomicAdd(P.counters + 4, 1ull);
+    }
+}
+
+
+// ------------------------------------------------------------------ plain form
+template <bool FP64, bool COUNT, bool DBG, bool LIST, int TM>
+__global__ void __launch_bounds__(256, OXR_PLAIN_WAVES) k_simulate(Params P) {
+    extern __shared__ int4 smem4[];
+    BNode* top = reinterpret_cast<BNode*>(smem4);
+    int* stk = reinterpret_cast<int*>(top + P.ntop) + threadIdx.x;
+    stage_top(P, top, P.ntop);
+    __syncthreads();
+    long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long steps = 0, visits = 0, tests = 0, csgb = 0, ivis = 0;
+    uint32_t err = 0;
+    if (idx < P.num_photon) {
+        Lane L;
+        if (start_photon<DBG>(P, L, idx)) {
+            uint32_t v = 0, t = 0, cb = 0, iv = 0;
+            while (step_photon<FP64, COUNT, DBG, LIST, TM>(P, L, stk, blockDim.x, top, P.ntop, v, t, err, cb, iv)) {}
+            end_photon<DBG>(P, L);
+            steps = (unsigned long long)L.bounce; visits = v; tests = t; csgb = cb; ivis = iv;
+        } else {
+            P.hitflag[idx] = 0;
+        }
+    }
+    flush_counters(P, steps, visits, tests, err, COUNT, csgb, ivis);
+}
+
+// ------------------------------------------------------------------ persistent form
+// Lanes hold one photon each and run one bounce step per loop turn. When at least
+// refill_threshold lanes of a wave are idle (or all are), the wave claims that many
+// photon indices with one atomicAdd on counters[1] and the idle lanes generate them.
+// Every wave leaves the loop once the counter has passed num_photon and its lanes are idle.
+template <bool FP64, bool DBG, bool LIST, int TM>
+__global__ void __launch_bounds__(256, OXR_PERSIST_WAVES) k_simulate_persist(Params P) {
+    extern __shared__ int4 smem4[];
+    BNode* top = reinterpret_cast<BNode*>(smem4);
+    int* stk = reinterpret_cast<int*>(top + P.ntop) + threadIdx.x;
+    stage_top(P, top, P.ntop);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    Lane L;
+    bool active = false;
+    bool drained = false;
+    unsigned long long steps = 0;
+    uint32_t v = 0, t = 0, err = 0, cb = 0, iv = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        int nidle = __popcll(idle);
+        if (!drained && nidle > 0 && (nidle >= P.refill_threshold || nidle == 64)) {
+            int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(P.counters + 1, (unsigned long long)nidle);
+            base = __shfl(base, leader, 64);
+            if (base + (unsigned long long)nidle >= (unsigned long long)P.num_photon) drained = true;
+            if (!active) {
+                unsigned rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
+                unsigned long long my = base + rank;
+                if (my < (unsigned long long)P.num_photon) {
+                    if (start_photon<DBG>(P, L, (long long)my)) active = true;
+                    else P.hitflag[my] = 0;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (drained) break;
+            continue;
+        }
+        if (active) {
+            if (!step_photon<FP64, false, DBG, LIST, TM>(P, L, stk, blockDim.x, top, P.ntop, v, t, err, cb, iv)) {
+                end_photon<DBG>(P, L);
+                steps += (unsigned long long)L.bounce;
+                active = false;
+            }
+        }
+    }
+    flush_counters(P, steps, 0, 0, err, false);
+}
+
+} // namespace oxd
+
+namespace oxl {
+
+// kind: SIM_PROD (production), SIM_DEBUG (seqhis, records, precooked streams), SIM_COUNT
+// (visit counting, generating in place like the debug variants). Debug launches use the
+// node-index stack (tm bit 0 clear).
+template <bool FP64, bool LIST>
+hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, int kind, int tm) {
+#define OXR_PLAIN(C, D, T) hipLaunchKernelGGL((oxd::k_simulate<FP64, C, D, LIST, T>), g, b, sm, s, P)
+    if (kind == SIM_COUNT) {
+        switch (tm) {
+            case 0: OXR_PLAIN(true, true, 0); break;
+            case 1: OXR_PLAIN(true, true, 1); break;
+            case 2: OXR_PLAIN(true, true, 2); break;
+            default: OXR_PLAIN(true, true, 3); break;
+        }
+    } else if (kind == SIM_DEBUG) {
+        if (tm & 2) OXR_PLAIN(false, true, 2);
+        else OXR_PLAIN(false, true, 0);
+    } else {
+        switch (tm) {
+            case 0: OXR_PLAIN(false, false, 0); break;
+            case 1: OXR_PLAIN(false, false, 1); break;
+            case 2: OXR_PLAIN(false, false, 2); break;
+            default: OXR_PLAIN(false, false, 3); break;
+        }
+    }
+#undef OXR_PLAIN
+    return hipGetLastError();
+}
+
+template <bool FP64, bool LIST>
+hipError_t launch_simulate_persist(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool debug, int tm) {
+#define OXR_PERSIST(D, T) hipLaunchKernelGGL((oxd::k_simulate_persist<FP64, D, LIST, T>), g, b, sm, s, P)
+    if (debug) {
+        if (tm & 2) OXR_PERSIST(true, 2);
+        else OXR_PERSIST(true, 0);
+    } else {
+        switch (tm) {
+            case 0: OXR_PERSIST(false, 0); break;
+            case 1: OXR_PERSIST(false, 1); break;
+            case 2: OXR_PERSIST(false, 2); break;
+            default: OXR_PERSIST(false, 3); break;
+        }
+    }
+#undef OXR_PERSIST
+    return hipGetLastError();
+}
+
+} // namespace oxl

@@ -1,0 +1,7 @@
+// oxr_sim_f64.hip -- simulate kernels with double-precision roots
+#include "oxr_sim.h"
+
+namespace oxl {
+template hipError_t launch_simulate_plain<true, false>(hipStream_t, const oxd::Params&, dim3, dim3, size_t, int, int);
+template hipError_t launch_simulate_persist<true, false>(hipStream_t, const oxd::Params&, dim3, dim3, size_t, bool, int);
+} // namespace oxl

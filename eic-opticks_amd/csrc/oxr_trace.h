@@ -11,7 +11,8 @@
 // A child ref >= 0 is an internal node, < 0 is a leaf item ~ref (one instance per TLAS
 // leaf, one prim per BLAS leaf). The traversal stack lives in LDS, one column per lane.
 //   flat[]  int2 (instance, local prim) per leaf when the geometry is flattened: one BVH
-//           over every placed prim's world box, no BLAS level (see Engine create).
+//           over every placed prim's world box, no BLAS level. trace<..., FLAT=true>
+//           walks that tree; FLAT=false walks the two-level one.
 #pragma once
 #include "oxr_device.h"
 
@@ -84,7 +85,7 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 // During traversal only (t, instance, local prim, object-frame normal) of the best hit
 // stay live; its world normal, local position and identity words are formed afterwards
 // from the instance record, with the same arithmetic as at instance entry.
-template <bool FP64, bool COUNT, bool LIST, bool TN = false>
+template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
@@ -144,7 +145,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             continue;
         }
         int item = ~cur;
-        if (g.flat) {
+        if (FLAT) {
             // flattened leaf: one placed prim, tested in its instance's frame; the same
             // (instance, local prim) key and arithmetic as the two-level path
             const int2 fi = g.flat[item];
