@@ -23,6 +23,7 @@ int hip_fail(hipError_t e, const char* what) {
 
 void EventBuffers::release_all() {
     gs.release(); gs_off.release(); seed.release(); photon.release(); seq.release(); record.release();
+    rec.release(); prd.release();
     hitflag.release(); hit.release(); counts.release(); offs.release(); counters.release();
     input.release(); pre.release(); lite.release(); hitlite.release(); merged.release(); rstate.release();
 }
@@ -74,6 +75,11 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.seq = cfg.seq_enabled ? b.seq.p : nullptr;
     P.record = cfg.max_record > 0 ? b.record.p : nullptr;
     P.max_record = cfg.max_record;
+    P.rec = cfg.max_rec > 0 ? b.rec.p : nullptr;
+    P.max_rec = cfg.max_rec;
+    P.prd = cfg.max_prd > 0 ? b.prd.p : nullptr;
+    P.max_prd = cfg.max_prd;
+    P.extent_domain = cfg.extent_domain; P.time_domain = cfg.time_domain;
     P.lite = cfg.photonlite ? b.lite.p : nullptr;
     P.rstate = b.rstate.p;
     P.hitflag = b.hitflag.p;
@@ -102,6 +108,8 @@ int ensure_event_capacity(Engine& e, EventBuffers& b, int64_t n) {
     OXR_HIP(b.photon.reserve(4 * N));
     if (e.cfg.seq_enabled) OXR_HIP(b.seq.reserve(2 * N));
     if (e.cfg.max_record > 0) OXR_HIP(b.record.reserve(4 * N * size_t(e.cfg.max_record)));
+    if (e.cfg.max_rec > 0) OXR_HIP(b.rec.reserve(N * size_t(e.cfg.max_rec)));
+    if (e.cfg.max_prd > 0) OXR_HIP(b.prd.reserve(2 * N * size_t(e.cfg.max_prd)));
     if (e.cfg.photonlite) OXR_HIP(b.lite.reserve(N));
     OXR_HIP(b.hitflag.reserve(N));
     OXR_HIP(b.rstate.reserve(N));
@@ -168,6 +176,8 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     OXR_HIP(hipMemsetAsync(b.counters.p, 0, 8 * sizeof(unsigned long long), s));
     // record slots past a photon's last point stay zero, as the reference's zeroed record array
     if (P.record && n > 0) OXR_HIP(hipMemsetAsync(P.record, 0, size_t(n) * size_t(e.cfg.max_record) * 64, s));
+    if (P.rec && n > 0) OXR_HIP(hipMemsetAsync(P.rec, 0, size_t(n) * size_t(e.cfg.max_rec) * 16, s));
+    if (P.prd && n > 0) OXR_HIP(hipMemsetAsync(P.prd, 0, size_t(n) * size_t(e.cfg.max_prd) * 32, s));
     OXR_HIP(hipEventRecord(e.e0, s));
     bool gen = oxl::needs_generate(P, count);
     if (gen) OXR_HIP(oxl::generate(s, P));
@@ -325,6 +335,13 @@ extern "C" {
 const char* oxr_version(void) { return "oxr 0.1 gfx950"; }
 const char* oxr_last_error(void) { return g_err.c_str(); }
 
+static bool config_ok(const oxr_config& c) {
+    if (c.max_record < 0 || c.max_photon < 1 || c.max_rec < 0 || c.max_prd < 0) return false;
+    // the rec domains divide each compressed coordinate
+    if (c.max_rec > 0 && !(c.extent_domain > 0.f && c.time_domain > 0.f)) return false;
+    return true;
+}
+
 void oxr_config_defaults(oxr_config* c) {
     if (!c) return;
     std::memset(c, 0, sizeof(*c));
@@ -337,7 +354,10 @@ void oxr_config_defaults(oxr_config* c) {
     c->rng_schedule = OXR_RNG_PRODUCTION;
     c->fp64_roots = 0; c->seq_enabled = 0; c->max_record = 0;
     c->max_photon = int64_t(1) << 24;
-    c->photonlite = 0; c->reserved = 0;
+    c->photonlite = 0;
+    c->max_rec = 0; c->max_prd = 0;
+    c->extent_domain = 1000.f; c->time_domain = 10.f;   // SEventConfig.cc:72-73
+    c->reserved = 0;
 }
 
 int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_config* cfg, oxr_engine** out) {
@@ -351,7 +371,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
         return fail(OXR_EINVAL, "physics needs bnd and optical tables");
     if (ph->icdf && (ph->icdf_rows < 1 || ph->icdf_width < 2)) return fail(OXR_EINVAL, "icdf shape");
     if (ph->icdf && ph->hd_factor != 0 && ph->icdf_rows < 3) return fail(OXR_EINVAL, "hd_factor needs 3 icdf rows");
-    if (cfg->max_record < 0 || cfg->max_photon < 1) return fail(OXR_EINVAL, "config");
+    if (!config_ok(*cfg)) return fail(OXR_EINVAL, "config");
     int dev = 0;
     int ndev = 0;
     OXR_HIP(hipGetDeviceCount(&ndev));
@@ -584,7 +604,7 @@ void oxr_engine_destroy(oxr_engine* eh) {
 int oxr_engine_set_config(oxr_engine* eh, const oxr_config* c) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !c) return fail(OXR_EINVAL, "null argument");
-    if (c->max_record < 0 || c->max_photon < 1) return fail(OXR_EINVAL, "config");
+    if (!config_ok(*c)) return fail(OXR_EINVAL, "config");
     std::lock_guard<std::mutex> lk(e->mu);
     e->cfg = *c;
     e->launched = false;
@@ -736,6 +756,12 @@ int oxr_event_get(oxr_engine* eh, int32_t what, void* dst, int64_t cap, int64_t*
         case OXR_GET_RECORD:
             if (e->cfg.max_record <= 0) return fail(OXR_ESTATE, "max_record is 0");
             src = e->ev.record.p; n = e->num_photon * e->cfg.max_record; item = 64; break;
+        case OXR_GET_REC:
+            if (e->cfg.max_rec <= 0) return fail(OXR_ESTATE, "max_rec is 0");
+            src = e->ev.rec.p; n = e->num_photon * e->cfg.max_rec; item = 16; break;
+        case OXR_GET_PRD:
+            if (e->cfg.max_prd <= 0) return fail(OXR_ESTATE, "max_prd is 0");
+            src = e->ev.prd.p; n = e->num_photon * e->cfg.max_prd; item = 32; break;
         case OXR_GET_SEED: src = e->ev.seed.p; n = e->num_photon; item = 4; break;
         case OXR_GET_GENSTEP: src = e->ev.gs.p; n = e->num_gs; item = 96; break;
         case OXR_GET_PHOTONLITE:

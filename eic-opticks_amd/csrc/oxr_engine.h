@@ -42,6 +42,8 @@ struct EventBuffers {
     DBuf<float4> photon;       // 4 float4 per photon
     DBuf<ulonglong2> seq;      // 2 per photon
     DBuf<float4> record;
+    DBuf<uint4> rec;           // srec per (photon, bounce) slot (config max_rec)
+    DBuf<float4> prd;          // quad2 per (photon, bounce) slot (config max_prd)
     DBuf<uint8_t> hitflag;
     DBuf<float4> hit;
     DBuf<int> counts;

@@ -33,6 +33,11 @@ struct Params {
     ulonglong2* seq;           // sseq out (2 x ulonglong2 per photon) or null
     float4* record;            // records or null
     int max_record;
+    uint4* rec;                // srec out (16 B per (photon, bounce) slot) or null
+    int max_rec;
+    float4* prd;               // quad2 prd out (2 float4 per slot) or null
+    int max_prd;
+    float extent_domain, time_domain;   // rec compression domains (mm, ns)
     uint4* lite;               // sphotonlite out (16 B per photon) or null
     ulonglong2* rstate;        // generate -> propagate hand-over: {next uniform, photon index};
                                // x = ~0 marks a padding slot of a batched step

@@ -59,6 +59,33 @@ OXD uint4 lite_pack(const Ph& p, float lposcost, float lposfphi) {
     return make_uint4(0x10000u | (p.identity & 0xffffu), __float_as_uint(p.time),
                       (unorm16(lposcost) << 16) | unorm16(lposfphi), p.flagmask);
 }
+// srec (sysrap/srec.h:39-43) as filled by sevent::add_rec (sysrap/sevent.h:614-622) with
+// the domains of sevent::init_domain (sevent.h:285-297): position and time over
+// centre-extent domains centred on zero as int16, polarization and wavelength as int8,
+// flag word zero. FLOAT2INT_RN (srec.h:106-110) is the round-to-nearest-even conversion,
+// saturating with NaN giving 0; its int result is narrowed to 16 or 8 bits by truncation,
+// so values outside a domain wrap round, as the reference notes (srec.h:117-120).
+OXD int f2i_rn(float x) {
+    if (!(x == x)) return 0;
+    const float r = rintf(x);
+    if (r >= 2147483648.f) return 2147483647;
+    if (r < -2147483647.f) return int(0x80000000u);
+    return int(r);
+}
+constexpr float REC_W_CENTER = 440.f;   // sevent w_center = (60 + 820) / 2 (sysrap/sevent.h:111-114)
+constexpr float REC_W_EXTENT = 380.f;   // sevent w_extent = (820 - 60) / 2
+OXD uint4 rec_pack(const Ph& p, float extent, float tdom) {
+    const int px = f2i_rn((32767.0f * (p.pos.x - 0.f)) / extent);
+    const int py = f2i_rn((32767.0f * (p.pos.y - 0.f)) / extent);
+    const int pz = f2i_rn((32767.0f * (p.pos.z - 0.f)) / extent);
+    const int pt = f2i_rn((32767.0f * (p.time - 0.f)) / tdom);
+    const int ax = f2i_rn(p.pol.x * 127.f), ay = f2i_rn(p.pol.y * 127.f), az = f2i_rn(p.pol.z * 127.f);
+    const int aw = f2i_rn(((p.wavelength - REC_W_CENTER) * 127.f) / REC_W_EXTENT);
+    return make_uint4((uint32_t(px) & 0xffffu) | (uint32_t(py) << 16), (uint32_t(pz) & 0xffffu) | (uint32_t(pt) << 16),
+                      (uint32_t(ax) & 0xffu) | ((uint32_t(ay) & 0xffu) << 8) | ((uint32_t(az) & 0xffu) << 16) |
+                          (uint32_t(aw) << 24),
+                      0u);
+}
 OXD void load_photon(Ph& p, const float4* s) {
     float4 a = s[0], b = s[1], c = s[2], e = s[3];
     p.pos = mk(a.x, a.y, a.z); p.time = a.w;

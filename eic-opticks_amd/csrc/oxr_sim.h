@@ -45,6 +45,7 @@ template <bool DBG>
 OXD void point(const Params& P, Lane& L) {
     if (!DBG) return;
     if (P.record && L.bounce < P.max_record) store_photon(P.record + 4 * (L.idx * P.max_record + L.bounce), L.p);
+    if (P.rec && L.bounce < P.max_rec) P.rec[L.idx * P.max_rec + L.bounce] = rec_pack(L.p, P.extent_domain, P.time_domain);
     if (P.seq) {
         unsigned slot = unsigned(L.bounce);
         if (slot < 32) {
@@ -113,9 +114,16 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
     }
     f3 normal = normalize(h.n);
     float lposcost = h.lpos.z / sqrtf(dot(h.lpos, h.lpos));
-    if (DBG && P.lite) {
+    if (DBG && (P.lite || P.prd)) {
         L.lc = lposcost;
         L.lf = (xatan2(h.lpos.y, h.lpos.x) + PIf) / (2.0f * PIf);   // normalize_fphi, sysrap/scuda.h:653
+    }
+    // sctx::trace (sysrap/sctx.h:170-173): the trace result of this bounce, normal normalised
+    // (CSGOptiX7.cu:466-474), in the quad2 layout of sysrap/squad.h:95-112
+    if (DBG && P.prd && L.bounce < P.max_prd) {
+        float4* q = P.prd + 2 * (L.idx * P.max_prd + L.bounce);
+        q[0] = make_float4(normal.x, normal.y, normal.z, h.t);
+        q[1] = make_float4(L.lc, L.lf, __uint_as_float(h.ii_id), __uint_as_float(h.gpbd));
     }
     int command = propagate(L.p, h, normal, lposcost, L.r, P.t, P.aligned);
     L.bounce++;

@@ -11,7 +11,7 @@ OXR_FORM_PERSISTENT = 1
 OXR_RNG_PRODUCTION = 0
 OXR_RNG_ALIGNED = 1
 (OXR_GET_PHOTON, OXR_GET_HIT, OXR_GET_SEQ, OXR_GET_RECORD, OXR_GET_SEED, OXR_GET_GENSTEP,
- OXR_GET_PHOTONLITE, OXR_GET_HITLITE) = range(8)
+ OXR_GET_PHOTONLITE, OXR_GET_HITLITE, OXR_GET_REC, OXR_GET_PRD) = range(10)
 
 # photon flags (sysrap/OpticksPhoton.h:22-46)
 FLAGS = dict(CERENKOV=1 << 0, SCINTILLATION=1 << 1, TORCH=1 << 2, BULK_ABSORB=1 << 3,
@@ -49,7 +49,9 @@ class Config(C.Structure):
                 ("hitmask", C.c_uint32), ("rng_seed", C.c_uint64), ("rng_offset", C.c_uint64),
                 ("skipahead_event", C.c_uint64), ("rng_schedule", C.c_int32),
                 ("fp64_roots", C.c_int32), ("seq_enabled", C.c_int32), ("max_record", C.c_int32),
-                ("max_photon", C.c_int64), ("photonlite", C.c_int32), ("reserved", C.c_int32)]
+                ("max_photon", C.c_int64), ("photonlite", C.c_int32), ("max_rec", C.c_int32),
+                ("max_prd", C.c_int32), ("extent_domain", C.c_float), ("time_domain", C.c_float),
+                ("reserved", C.c_int32)]
 
 
 class Tuning(C.Structure):
@@ -144,6 +146,10 @@ def default_config(**kw):
     c.max_record = 0
     c.max_photon = 1 << 24
     c.photonlite = 0
+    c.max_rec = 0
+    c.max_prd = 0
+    c.extent_domain = 1000.0    # SEventConfig MaxExtentDomain (mm)
+    c.time_domain = 10.0        # SEventConfig MaxTimeDomain (ns)
     c.reserved = 0
     for k, v in kw.items():
         setattr(c, k, v)

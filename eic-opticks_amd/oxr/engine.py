@@ -191,12 +191,18 @@ class Engine:
             abi.OXR_GET_GENSTEP: ((n.value, 6, 4), np.float32),
             abi.OXR_GET_PHOTONLITE: ((n.value, 4), np.uint32),
             abi.OXR_GET_HITLITE: ((n.value, 4), np.uint32),
+            abi.OXR_GET_REC: ((n.value, 4), np.uint32),
+            abi.OXR_GET_PRD: ((n.value, 2, 4), np.float32),
         }[what]
         out = np.zeros(shape, dtype=dt)
         m = C.c_int64()
         _check(lib().oxr_event_get(self.h, what, abi.ptr(out), n.value, C.byref(m)))
         if what == abi.OXR_GET_RECORD:
             out = out.reshape(self.num_photon, -1, 4, 4)
+        elif what == abi.OXR_GET_REC:
+            out = out.reshape(self.num_photon, -1, 4)
+        elif what == abi.OXR_GET_PRD:
+            out = out.reshape(self.num_photon, -1, 2, 4)
         return out
 
     def photons(self):

@@ -21,6 +21,9 @@
  *                   photon/hit sphoton (n,4,4)f32 (sysrap/sphoton.h:171-192)
  *                   seq sseq (n,2,2)u64 = seqhis[2], seqbnd[2] (sysrap/sseq.h:54-65)
  *                   record (n,max_record,4,4)f32 (sysrap/sctx.h:134)
+ *                   rec srec (n,max_rec), 16 B each: short4 post, char4 polw, uchar4 flag
+ *                       (sysrap/srec.h:39-43, sevent::add_rec sysrap/sevent.h:614-622)
+ *                   prd quad2 (n,max_prd,2,4)f32 (sysrap/sctx.h:170-173, squad.h:95-112)
  */
 #ifndef OXR_H
 #define OXR_H
@@ -73,8 +76,15 @@ enum {
     OXR_GET_GENSTEP= 5,   /* quad6[num_genstep]                                      */
     OXR_GET_PHOTONLITE = 6, /* sphotonlite[num_photon] (needs photonlite)
                                QEvt::gatherPhotonLite, sysrap/sphotonlite.h:47-52     */
-    OXR_GET_HITLITE    = 7  /* sphotonlite[num_hit]: the photonlite items of the hits,
+    OXR_GET_HITLITE    = 7, /* sphotonlite[num_hit]: the photonlite items of the hits,
                                QEvt::gatherHitLite (qudarap/QEvt.cc:985-1019)         */
+    OXR_GET_REC        = 8, /* srec[num_photon*max_rec] (needs max_rec > 0): the
+                               domain-compressed step points of sctx::point
+                               (sysrap/sctx.h:134-140, sevent::add_rec)               */
+    OXR_GET_PRD        = 9  /* quad2[num_photon*max_prd] (needs max_prd > 0): the
+                               trace result of each bounce with its normal normalised,
+                               sctx::trace (sysrap/sctx.h:170-173); slots of bounces
+                               that found no boundary stay zero                       */
 };
 
 /* ---- inputs ----------------------------------------------------------- */
@@ -129,6 +139,11 @@ typedef struct oxr_config {
     int32_t  photonlite;      /* 1: also write sphotonlite (16 B) per photon, the last
                                  intersect's lposcost/lposfphi packed (SEventConfig
                                  ModeLite, CSGOptiX7.cu:493-499); default 0           */
+    int32_t  max_rec;         /* >0: write per-bounce srec (SEventConfig MaxRec)      */
+    int32_t  max_prd;         /* >0: write per-bounce quad2 prd (SEventConfig MaxPrd) */
+    float    extent_domain;   /* rec position domain, mm, centred on the origin
+                                 (SEventConfig MaxExtentDomain, default 1000)         */
+    float    time_domain;     /* rec time domain, ns (MaxTimeDomain, default 10)      */
     int32_t  reserved;        /* 0                                                    */
 } oxr_config;
 

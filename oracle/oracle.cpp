@@ -1513,7 +1513,36 @@ struct Outputs {
     float* photon; uint64_t* seq; float* record; int32_t max_record;
     uint32_t* bounce; uint32_t* overflow;
     uint32_t* lite;     // sphotonlite (4 words) per photon or null
+    uint32_t* rec; int32_t max_rec;     // srec (4 words) per (photon, bounce) slot or null
+    float* prd; int32_t max_prd;        // quad2 (8 floats) per (photon, bounce) slot or null
 };
+
+// FLOAT2INT_RN (sysrap/srec.h:106-110): round to nearest even, saturating, NaN giving 0
+inline int32_t f2i_rn(float x) {
+    if (x != x) return 0;
+    const float r = std::nearbyint(x);
+    if (r >= 2147483648.f) return 2147483647;
+    if (r < -2147483647.f) return INT32_MIN;
+    return int32_t(r);
+}
+
+// sevent::add_rec (sysrap/sevent.h:614-622) with the srec setters (sysrap/srec.h:126-191)
+// and the domains of sevent::init_domain (sevent.h:285-297): centre zero, position extent
+// and time extent from the config, wavelength centre (60+820)/2 and extent (820-60)/2.
+// The int results are narrowed to int16 (post) and int8 (polw) by truncation.
+inline void rec_pack(uint32_t* r, float px, float py, float pz, float t, float ax, float ay, float az, float w,
+                     float extent, float tdom) {
+    const int32_t qx = f2i_rn((32767.0f * (px - 0.f)) / extent);
+    const int32_t qy = f2i_rn((32767.0f * (py - 0.f)) / extent);
+    const int32_t qz = f2i_rn((32767.0f * (pz - 0.f)) / extent);
+    const int32_t qt = f2i_rn((32767.0f * (t - 0.f)) / tdom);
+    const int32_t wx = f2i_rn(ax * 127.f), wy = f2i_rn(ay * 127.f), wz = f2i_rn(az * 127.f);
+    const int32_t ww = f2i_rn(((w - 440.f) * 127.f) / 380.f);
+    r[0] = (uint32_t(qx) & 0xffffu) | (uint32_t(qy) << 16);
+    r[1] = (uint32_t(qz) & 0xffffu) | (uint32_t(qt) << 16);
+    r[2] = (uint32_t(wx) & 0xffu) | ((uint32_t(wy) & 0xffu) << 8) | ((uint32_t(wz) & 0xffu) << 16) | (uint32_t(ww) << 24);
+    r[3] = 0u;
+}
 
 // sphotonlite::set_lpos (sysrap/sphotonlite.h:234-243): v*0xffff+0.5 converted to 16 bits;
 // the conversion saturates, negative and NaN giving 0.
@@ -1547,6 +1576,10 @@ void simulate_one(const Ctx& c, const float* gs_all, const int32_t* seed, int64_
     uint64_t seqhis[2] = {0, 0}, seqbnd[2] = {0, 0};
     auto point = [&](int bounce) {
         if (out.record && bounce < out.max_record) p.store(out.record + 16 * (size_t(idx) * out.max_record + bounce));
+        if (out.rec && bounce < out.max_rec)
+            rec_pack(out.rec + 4 * (size_t(idx) * out.max_rec + bounce), float(p.pos.x), float(p.pos.y), float(p.pos.z),
+                     float(p.time), float(p.pol.x), float(p.pol.y), float(p.pol.z), float(p.wavelength),
+                     c.cfg.extent_domain, c.cfg.time_domain);
         if (out.seq) {
             unsigned slot = unsigned(bounce);
             unsigned iseq = slot / 16;
@@ -1568,6 +1601,12 @@ void simulate_one(const Ctx& c, const float* gs_all, const int32_t* seed, int64_
         last_lposcost = prd.lposcost; last_lposfphi = prd.lposfphi;
         if ((prd.gp_boundary & 0xffffu) == 0xffffu) break;
         V3<T> normal = normalize(prd.normal);
+        if (out.prd && bounce < out.max_prd) {
+            // sctx::trace (sysrap/sctx.h:170-173): quad2 of sysrap/squad.h:95-112
+            float* q = out.prd + 8 * (size_t(idx) * out.max_prd + bounce);
+            q[0] = float(normal.x); q[1] = float(normal.y); q[2] = float(normal.z); q[3] = float(prd.t);
+            q[4] = prd.lposcost; q[5] = prd.lposfphi; q[6] = u2f(prd.iindex_identity); q[7] = u2f(prd.gp_boundary);
+        }
         int command = sim.propagate(p, prd, normal);
         bounce++;
         point(bounce);
@@ -1772,14 +1811,15 @@ int64_t orc_simulate(void* cv, const float* gs, int64_t num_gs, int32_t event_in
                      const float* precooked, int32_t npre, const float* input_photons,
                      int32_t precision, int32_t nthreads, int64_t photon_begin, int64_t photon_end,
                      float* photon, uint64_t* seq, float* record, int32_t max_record, uint32_t* bounce,
-                     uint32_t* overflow, uint32_t* lite) {
+                     uint32_t* overflow, uint32_t* lite, uint32_t* rec, int32_t max_rec, float* prd,
+                     int32_t max_prd) {
     const orc::Ctx& c = *static_cast<orc::Ctx*>(cv);
     int64_t total = orc_fill_seed(gs, num_gs, nullptr, 0);
     std::vector<int32_t> seed(static_cast<size_t>(total));
     orc_fill_seed(gs, num_gs, seed.data(), total);
     if (photon_end < 0 || photon_end > total) photon_end = total;
     if (photon_begin < 0) photon_begin = 0;
-    orc::Outputs out{photon, seq, record, max_record, bounce, overflow, lite};
+    orc::Outputs out{photon, seq, record, max_record, bounce, overflow, lite, rec, max_rec, prd, max_prd};
     std::atomic<int64_t> next(photon_begin);
     auto work = [&]() {
         for (;;) {

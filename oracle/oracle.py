@@ -47,7 +47,7 @@ def lib():
         L.orc_simtrace.restype = i64
         L.orc_simulate.restype = i64
         L.orc_simulate.argtypes = [vp, vp, i64, i32, u64, vp, i32, vp, i32, i32, i64, i64,
-                                   vp, vp, vp, i32, vp, vp, vp]
+                                   vp, vp, vp, i32, vp, vp, vp, vp, i32, vp, i32]
         L.orc_propagate_at_boundary.argtypes = [vp, C.c_float, C.c_float, vp, i32, i32, vp]
         L.orc_sample.argtypes = [i32, vp, i64, u64, vp]
         L.orc_boundary_lookup.argtypes = [vp, vp, i64, u32, u32, vp]
@@ -204,7 +204,8 @@ class Oracle:
         return out
 
     def simulate(self, gs, event_index=0, slot_offset=0, precooked=None, input_photons=None,
-                 precision=0, nthreads=8, begin=0, end=-1, seq=True, max_record=0, lite=False):
+                 precision=0, nthreads=8, begin=0, end=-1, seq=True, max_record=0, lite=False,
+                 max_rec=0, max_prd=0):
         gs = np.ascontiguousarray(gs, dtype=np.float32)
         total = len(fill_seed(gs))
         if end < 0 or end > total:
@@ -215,6 +216,8 @@ class Oracle:
         bounce = np.zeros(total, dtype=np.uint32)
         overflow = np.zeros(total, dtype=np.uint32)
         lt = np.zeros((total, 4), dtype=np.uint32) if lite else None
+        rc = np.zeros((total, max_rec, 4), dtype=np.uint32) if max_rec > 0 else None
+        pd = np.zeros((total, max_prd, 2, 4), dtype=np.float32) if max_prd > 0 else None
         pre = None
         npre = 0
         if precooked is not None:
@@ -223,11 +226,12 @@ class Oracle:
         inp = None if input_photons is None else np.ascontiguousarray(input_photons, dtype=np.float32)
         lib().orc_simulate(self.h, P(gs), len(gs), event_index, slot_offset, P(pre), npre, P(inp),
                            precision, nthreads, begin, end, P(photon), P(sq), P(rec), max_record,
-                           P(bounce), P(overflow), P(lt))
+                           P(bounce), P(overflow), P(lt), P(rc), max_rec, P(pd), max_prd)
         sl = slice(begin, end)
         return dict(photon=photon[sl], seq=None if sq is None else sq[sl],
                     record=None if rec is None else rec[sl], bounce=bounce[sl],
-                    overflow=overflow[sl], total=total, lite=None if lt is None else lt[sl])
+                    overflow=overflow[sl], total=total, lite=None if lt is None else lt[sl],
+                    rec=None if rc is None else rc[sl], prd=None if pd is None else pd[sl])
 
     def boundary_lookup(self, nm, line, k):
         nm = np.ascontiguousarray(nm, dtype=np.float32)

@@ -26,7 +26,7 @@ def test_every_declared_symbol_is_exported(engine_mod):
 def test_struct_sizes_match_header(engine_mod):
     # sizes implied by include/oxr.h on LP64
     assert C.sizeof(abi.Geometry) == 96
-    assert C.sizeof(abi.Config) == 88
+    assert C.sizeof(abi.Config) == 104
     assert C.sizeof(abi.LaunchStats) == 88
     assert C.sizeof(abi.Tuning) == 24
     assert C.sizeof(abi.RequestResult) == 64
@@ -44,6 +44,8 @@ def test_config_defaults_are_reference_defaults(engine_mod):
     assert c.skipahead_event == 100000
     assert c.hitmask == F['SURFACE_DETECT']
     assert c.max_time > 1e26
+    # rec domains (SEventConfig.cc:72-73): 1000 mm, 10 ns; rec, prd and records off
+    assert c.extent_domain == 1000.0 and c.time_domain == 10.0 and c.max_rec == 0 and c.max_prd == 0
     py = abi.default_config()
     for k, _ in abi.Config._fields_:
         assert getattr(py, k) == getattr(c, k), k
