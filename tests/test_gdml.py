@@ -360,6 +360,22 @@ def test_pfrich_loads_and_simulates(oracle_lib):
     assert ((fm & F['SURFACE_DETECT']) != 0).sum() > 0 and ((fm & F['BOUNDARY_TRANSMIT']) != 0).mean() > 0.9
 
 
+def test_raindrop_simg4ox_geometry(oracle_lib):
+    """tests/geom/raindrop.gdml (tests/test_simg4ox.sh): a water drop in air inside a lead
+    container with a border surface from the air to the lead. Lead has no RINDEX, so the
+    surface lands on the inner side only; photons from the drop's centre leave the water
+    and are absorbed at the container."""
+    g = gdml.load_gdml(geom('raindrop'))
+    assert g.prim_volumes == ['VACUUM_lv', 'container_pv', 'medium_pv', 'drop_pv']
+    assert g.boundaries == ['VACUUM///VACUUM', 'VACUUM///G4_Pb', 'G4_Pb//medium_container_bs/G4_AIR',
+                            'G4_AIR///G4_WATER']
+    o = oracle_lib.Oracle(g.geometry, g.physics, abi.default_config(max_bounce=32))
+    r = o.simulate(G.torch([5000] * 2, pos=(0.0, 0.0, 0.0), type=G.T_POINT_ISOTROPIC, wavelength=420.0))
+    fm = flags(r)
+    assert ((fm & F['BOUNDARY_TRANSMIT']) != 0).mean() > 0.9
+    assert ((fm & F['SURFACE_ABSORB']) != 0).mean() > 0.8
+
+
 @pytest.mark.parametrize('name', ['sphere_leak', 'basic_detector', 'opticks_raindrop_with_scintillation'])
 def test_other_reference_geometries(oracle_lib, name):
     g = gdml.load_gdml(geom(name))

@@ -62,6 +62,17 @@ def test_raindrop_photon_file_kat(engine_mod):
     assert sorted(wl.tolist()) == [420] * 3 + [450] * 2 + [500] * 5
 
 
+@pytest.mark.parametrize('form', [abi.OXR_FORM_PLAIN, abi.OXR_FORM_PERSISTENT])
+def test_raindrop_simg4ox_bitexact(engine_mod, oracle_lib, form):
+    """tests/geom/raindrop.gdml (tests/test_simg4ox.sh geometry), isotropic photons from the drop."""
+    g = gdml.load_gdml(geom('raindrop'))
+    gs = G.torch([5000] * 2, pos=(0.0, 0.0, 0.0), type=G.T_POINT_ISOTROPIC, wavelength=420.0)
+    print('raindrop GDML, 10000 photons, form', form, flush=True)
+    ph, _ = run_gpu(engine_mod, g, gs, form, max_bounce=32)
+    want = run_oracle(oracle_lib, g, gs, max_bounce=32)
+    assert np.array_equal(ph.view(np.uint32), want.view(np.uint32))
+
+
 @pytest.mark.parametrize('name', ['sphere_leak', 'basic_detector'])
 def test_other_geometries_bitexact(engine_mod, oracle_lib, name):
     g = gdml.load_gdml(geom(name))
