@@ -329,7 +329,11 @@ OXD int propagate_at_surface(unsigned& flag, Ph& p, const St& s, f3 normal, Rng&
 }
 
 // qsim::propagate (qudarap/qsim.h:2218-2327)
-OXD int propagate(Ph& p, const Hit& h, f3 normal, float lposcost, Rng& r, const Tables& t, int aligned) {
+// normalize_cost (sysrap/scuda.h): cos(theta) of the object-frame hit position
+OXD float lpos_cost(f3 lpos) { return lpos.z / sqrtf(dot(lpos, lpos)); }
+
+// The hit's lposcost is formed only on the branch that reads it.
+OXD int propagate(Ph& p, const Hit& h, f3 normal, Rng& r, const Tables& t, int aligned) {
     unsigned boundary = h.gpbd & 0xffffu;
     unsigned identity = h.ii_id & 0xffffu;
     unsigned iindex = h.ii_id >> 16;
@@ -344,7 +348,7 @@ OXD int propagate(Ph& p, const Hit& h, f3 normal, float lposcost, Rng& r, const 
         int ems = int(s.opt.y);
         if (ems == 1) command = propagate_at_boundary(flag, p, s, normal, r, aligned);
         else if (ems == 2) command = propagate_at_surface(flag, p, s, normal, r, aligned);
-        else if (lposcost < 0.f) command = propagate_at_surface(flag, p, s, normal, r, aligned);
+        else if (lpos_cost(h.lpos) < 0.f) command = propagate_at_surface(flag, p, s, normal, r, aligned);
         else if (ems == 3) { uniform(r); flag = SURFACE_DETECT; command = BREAK; }
     }
     set_flag(p, flag);

@@ -112,10 +112,12 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
         if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0
         return false;
     }
-    f3 normal = normalize(h.n);
-    float lposcost = h.lpos.z / sqrtf(dot(h.lpos, h.lpos));
+    // normalize(n) is n * (1 / sqrt(dot(n, n))): when that dot rounds to exactly 1 it
+    // returns n itself, so an already unit normal (boxes, polyhedra) skips the arithmetic
+    const float nn = dot(h.n, h.n);
+    f3 normal = nn == 1.f ? h.n : normalize(h.n);
     if (DBG && (P.lite || P.prd)) {
-        L.lc = lposcost;
+        L.lc = lpos_cost(h.lpos);
         L.lf = (xatan2(h.lpos.y, h.lpos.x) + PIf) / (2.0f * PIf);   // normalize_fphi, sysrap/scuda.h:653
     }
     // sctx::trace (sysrap/sctx.h:170-173): the trace result of this bounce, normal normalised
@@ -125,7 +127,7 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
         q[0] = make_float4(normal.x, normal.y, normal.z, h.t);
         q[1] = make_float4(L.lc, L.lf, __uint_as_float(h.ii_id), __uint_as_float(h.gpbd));
     }
-    int command = propagate(L.p, h, normal, lposcost, L.r, P.t, P.aligned);
+    int command = propagate(L.p, h, normal, L.r, P.t, P.aligned);
     L.bounce++;
     point<DBG>(P, L);
     return command != BREAK;
