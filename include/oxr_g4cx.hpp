@@ -9,7 +9,9 @@
  *       GetNumPhotonCollected / getSimtrace (sysrap/SEvt.hh), one GPU event store;
  *   U4-style genstep makers taking the step values U4::CollectGenstep_* reads from
  *       G4Track/G4Step (u4/U4.cc:150-330) as plain numbers;
- *   load_photons_txt (src/GPUPhotonFileSource.h:47-88).
+ *   load_photons_txt (src/GPUPhotonFileSource.h:47-88);
+ *   Simulator<Base>: the launcher protocol of sysrap/SSimulator.h:16-37 (simulate_launch,
+ *       simtrace_launch, launch, desc, simulate, simtrace, reset) over one engine.
  *
  * Header only; link with liboxr.so. Errors from the C ABI throw std::runtime_error with
  * oxr_last_error(). Environment read at SetGeometry, as SEventConfig reads it:
@@ -21,6 +23,7 @@
 #include "oxr.h"
 
 #include <cctype>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -294,6 +297,93 @@ private:
     oxr_foundry* foundry_ = nullptr;
     oxr_engine* engine_ = nullptr;
     oxr_config cfg_{};
+    oxr_launch_stats last_{};
+};
+
+/* ---- Simulator: the launcher protocol of sysrap/SSimulator.h:16-37 -----------------
+ * Base is the interface the host is built against: the reference's SSimulator, whose
+ * pure virtual members these then override (oxr::Simulator<SSimulator>), or NoBase to
+ * use the class on its own. simulate and simtrace take the gensteps of
+ * SEvt::Get_EGPU() (the role QEvt plays for CSGOptiX, qudarap/QSim.cc:488-504) and leave
+ * hits or simtrace items there; simulate_launch relaunches the engine's current event.
+ * Every call returns the launch's wall seconds, as the reference's launch timing does.
+ * Rendering (the OptiX render pipeline) is not part of this engine: render and
+ * render_launch throw std::logic_error. The engine is borrowed, not owned. */
+struct NoBase {
+    virtual ~NoBase() = default;
+};
+
+template <class Base = NoBase>
+class Simulator : public Base {
+public:
+    enum Mode { SIMULATE, SIMTRACE };   /* what launch() runs, the role of SEventConfig RGMode */
+
+    explicit Simulator(oxr_engine* e, int32_t form = OXR_FORM_PERSISTENT) : engine_(e), form_(form) {
+        if (!e) throw std::invalid_argument("oxr::Simulator needs an engine");
+        desc_ = std::string(oxr_version()) + " : simulate and simtrace on HIP, "
+                + (form == OXR_FORM_PERSISTENT ? "persistent" : "plain") + " form";
+    }
+
+    /* low level: launches over the engine's current event */
+    double render_launch() { throw std::logic_error("oxr::Simulator: rendering is not part of this engine"); }
+    double simtrace_launch() { return simtrace(event_); }
+    double simulate_launch() {
+        oxr_launch_stats st{};
+        check(oxr_simulate_launch(engine_, form_, &st), "simulate launch");
+        last_ = st;
+        return st.total_ms * 1e-3;
+    }
+    double launch() { return mode_ == SIMTRACE ? simtrace_launch() : simulate_launch(); }
+    const char* desc() const { return desc_.c_str(); }
+
+    /* high level: the SEvt's gensteps in, hits (or simtrace items) out */
+    double simulate(int eventID, bool reset_ = false) {
+        SEvt* ev = SEvt::Get_EGPU();
+        event_ = eventID;
+        if (!ev->input_.empty())
+            check(oxr_event_set_input_photons(engine_, reinterpret_cast<const float*>(ev->input_.data()),
+                                               int64_t(ev->input_.size())), "set input photons");
+        int64_t n = 0;
+        check(oxr_event_set_gensteps(engine_, reinterpret_cast<const float*>(ev->gensteps_.data()),
+                                     int64_t(ev->gensteps_.size()), eventID, 0, &n), "set gensteps");
+        const double s = simulate_launch();
+        int64_t nh = 0;
+        int rc = oxr_event_get(engine_, OXR_GET_HIT, nullptr, 0, &nh);
+        if (rc != OXR_OK && rc != OXR_ECAPACITY) check(rc, "hit count");
+        ev->hits_.assign(size_t(nh), sphoton{});
+        if (nh > 0) check(oxr_event_get(engine_, OXR_GET_HIT, ev->hits_.data(), nh, &nh), "gather hits");
+        if (reset_) reset(eventID);
+        return s;
+    }
+    double simtrace(int eventID) {
+        SEvt* ev = SEvt::Get_EGPU();
+        event_ = eventID;
+        const float* gs = reinterpret_cast<const float*>(ev->gensteps_.data());
+        const auto t0 = std::chrono::steady_clock::now();
+        int64_t n = 0;
+        int rc = oxr_simtrace(engine_, gs, int64_t(ev->gensteps_.size()), nullptr, 0, &n);
+        if (rc != OXR_OK && rc != OXR_ECAPACITY) check(rc, "simtrace");
+        ev->simtrace_.assign(size_t(n), simtrace_item{});
+        check(oxr_simtrace(engine_, gs, int64_t(ev->gensteps_.size()), reinterpret_cast<float*>(ev->simtrace_.data()),
+                           n, &n), "simtrace");
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    double render(const char* = nullptr) { return render_launch(); }
+    void reset(int) {
+        SEvt* ev = SEvt::Get_EGPU();
+        ev->gensteps_.clear();
+        ev->input_.clear();
+    }
+
+    void set_mode(Mode m) { mode_ = m; }
+    const oxr_launch_stats& last_launch() const { return last_; }
+
+private:
+    oxr_engine* engine_;
+    int32_t form_;
+    int event_ = 0;
+    Mode mode_ = SIMULATE;
+    std::string desc_;
     oxr_launch_stats last_{};
 };
 

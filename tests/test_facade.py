@@ -99,3 +99,35 @@ def test_example_argument_errors(engine_mod):
     r = subprocess.run([EXE, '-p', os.path.join(GOLDEN, 'raindrop_photons.txt'), '-g', '/nonexistent/cf'],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and 'no CSGFoundry folder' in r.stderr
+
+
+# The pure virtual members of sysrap/SSimulator.h:16-37, as a host built against that
+# interface declares them.
+SSIMULATOR_LIKE = r'''
+struct SSimulatorLike {
+    virtual ~SSimulatorLike() = default;
+    virtual double render_launch() = 0;
+    virtual double simtrace_launch() = 0;
+    virtual double simulate_launch() = 0;
+    virtual double launch() = 0;
+    virtual const char* desc() const = 0;
+    virtual double simulate(int eventID, bool reset = false) = 0;
+    virtual double simtrace(int eventID) = 0;
+    virtual double render(const char* stem = nullptr) = 0;
+    virtual void reset(int eventID) = 0;
+};
+'''
+
+
+def test_simulator_implements_the_launcher_protocol(tmp_path):
+    """oxr::Simulator<Base> overrides every pure virtual of the launcher interface, so a
+    host can hold it through that base (compiled only: no engine is created here)."""
+    src = tmp_path / 'sim_iface.cpp'
+    src.write_text('#include "oxr_g4cx.hpp"\n#include <type_traits>\n' + SSIMULATOR_LIKE + r'''
+static_assert(!std::is_abstract<oxr::Simulator<SSimulatorLike>>::value, "all launcher members implemented");
+static_assert(std::is_base_of<SSimulatorLike, oxr::Simulator<SSimulatorLike>>::value, "held through the interface");
+static_assert(!std::is_abstract<oxr::Simulator<>>::value, "usable on its own");
+int main() { return 0; }
+''')
+    subprocess.check_call(['g++', '-std=c++17', '-Wall', '-Werror', '-fsyntax-only', '-I', os.path.join(ROOT, 'include'),
+                           str(src)])

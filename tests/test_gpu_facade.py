@@ -10,7 +10,7 @@ import pytest
 
 from oxr import cfio, gdml
 
-from test_facade import EXE, GOLDEN
+from test_facade import EXE, GOLDEN, ROOT as ROOT_DIR
 from test_gdml import geom
 
 pytestmark = pytest.mark.gpu
@@ -56,3 +56,39 @@ def test_comments_and_blank_lines(raindrop_folder, tmp_path):
 def test_missing_photons_argument_fails(raindrop_folder, tmp_path):
     r = run(['-g', raindrop_folder], str(tmp_path))
     assert r.returncode != 0
+
+
+def test_simulator_through_the_launcher_interface(raindrop_folder, tmp_path):
+    """The raindrop known-answer case (10 input photons, 10 hits) driven through a pointer
+    to the launcher interface of sysrap/SSimulator.h, with oxr::Simulator behind it."""
+    from test_facade import SSIMULATOR_LIKE
+    src = tmp_path / 'sim.cpp'
+    src.write_text('#include "oxr_g4cx.hpp"\n#include <cstdio>\n' + SSIMULATOR_LIKE + r'''
+int main(int argc, char** argv) {
+    oxr::G4CXOpticks* gx = oxr::G4CXOpticks::SetGeometry(argv[1]);
+    oxr::Simulator<SSimulatorLike> impl(gx->engine());
+    SSimulatorLike* sim = &impl;
+    oxr::SEvt* ev = oxr::SEvt::Get_EGPU();
+    ev->SetInputPhoton(oxr::load_photons_txt(argv[2]));
+    double s = sim->simulate(0, true);
+    std::printf("hits %u seconds_positive %d gensteps_after_reset %zu\n", ev->GetNumHit(0), s > 0.0 ? 1 : 0,
+                ev->gensteps_.size());
+    std::printf("desc %s\n", sim->desc());
+    int threw = 0;
+    try { sim->render_launch(); } catch (const std::logic_error&) { threw = 1; }
+    std::printf("render_throws %d\n", threw);
+    oxr::G4CXOpticks::Finalize();
+    return 0;
+}
+''')
+    exe = tmp_path / 'sim'
+    lib = os.path.dirname(EXE)
+    subprocess.check_call(['g++', '-std=c++17', '-Wall', '-I', os.path.join(ROOT_DIR, 'include'), str(src), '-o', str(exe),
+                           '-L', lib, '-loxr', f'-Wl,-rpath,{lib}'])
+    print('SSimulator-interface program on the raindrop folder', flush=True)
+    r = subprocess.run([str(exe), raindrop_folder, os.path.join(GOLDEN, 'raindrop_photons.txt')], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = dict(l.split(' ', 1) for l in r.stdout.strip().splitlines())
+    assert out['hits'] == '10 seconds_positive 1 gensteps_after_reset 0'
+    assert out['desc'].startswith('oxr') and out['render_throws'] == '1'
