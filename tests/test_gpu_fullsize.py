@@ -61,3 +61,32 @@ def test_instanced_csg_full(engine_mod, oracle_lib):
     assert n == 10_000_000
     _, st = _check_event(engine_mod, oracle_lib, s, gs, cfg, [(0, 400), (5_000_000, 5_000_400), (n - 400, n)])
     assert st['num_hit'] > 0.05 * n
+
+
+def test_calorimeter_production_whole_event_bitexact(engine_mod, oracle_lib):
+    """The headline configuration itself: production kernels (no debug outputs), the
+    committed tuning for the workload (persistent form, flattened BVH, distance stack),
+    every one of the event's photons against the oracle, and the hits it selects."""
+    import json
+    import os
+    table = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'eic-opticks_amd', 'oxr',
+                         'tuning_gfx950.json')
+    with open(table) as f:
+        tune = json.load(f)['csi_calorimeter']
+    s = scenes.csi_calorimeter()
+    cfg = abi.default_config(**s.config)
+    gs = s.gensteps()
+    print('whole calorimeter event, production kernels, tuning', tune, flush=True)
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        e.set_tuning(**tune)
+        n = e.set_gensteps(gs)
+        st = e.launch(abi.OXR_FORM_PERSISTENT)
+        ph, hit = e.photons(), e.hits()
+    assert n > 2_000_000 and len(ph) == n
+    print('oracle on', n, 'photons', flush=True)
+    want = oracle_lib.Oracle(s.geometry, s.physics, cfg).simulate(gs, nthreads=min(16, os.cpu_count() or 1),
+                                                                  seq=False)['photon']
+    assert np.array_equal(bits(ph), bits(want))
+    sel = (bits(want)[:, 3, 3] & cfg.hitmask) == cfg.hitmask
+    assert st['num_hit'] == len(hit) == int(sel.sum()) > 0
+    assert np.array_equal(bits(hit), bits(want[sel]))
