@@ -256,6 +256,31 @@ def run_sequential(e, evs, arrivals):
     return lat, time.perf_counter() - t0, photons, busy
 
 
+def instanced_only(engine_mod, abi, scenes, args):
+    """The instanced-CSG event alone (persistent then plain launches, committed tuning),
+    so that a kernel trace of this command holds only its 10M-photon launches."""
+    s3 = scenes.instanced_csg()
+    cfg3 = abi.default_config(**s3.config)
+    with engine_mod.Engine(s3.geometry, s3.physics, cfg3) as e3:
+        t3 = load_tuning('instanced_csg')
+        if t3:
+            e3.set_tuning(**t3)
+        n3 = e3.set_gensteps(s3.gensteps())
+        dt, st, k = timed_launches(e3, abi.OXR_FORM_PERSISTENT, args.steps, args.warmup)
+        dtp, _, kp = timed_launches(e3, abi.OXR_FORM_PLAIN, args.steps, args.warmup)
+        tuning = e3.tuning()
+    out = {'metric': 'photons/s on 1 MI355X (instanced-CSG event)', 'value': round(args.steps * n3 / dt, 1),
+           'unit': 'photons/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+           'ms_per_step': round(1e3 * dt / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
+           'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+           'config': {'workload': 'instanced_csg: water sphere r=19.5 m, acrylic shell, 20000 instanced 7-node '
+                                  'CSG solids, 10M torch photons', 'photons': n3, 'form': 'persistent',
+                      'tuning': tuning},
+           'kernel_ms': round(k, 3), 'bounce_steps': int(st['bounce_steps']),
+           'plain_form': {'photons_per_s': round(args.steps * n3 / dtp, 1), 'kernel_ms': round(kp, 3)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -265,6 +290,9 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
     ap.add_argument('--no-precision', action='store_true', help='skip the thin-shell precision and history-chi2 legs')
+    ap.add_argument('--workload', choices=['csi_calorimeter', 'instanced_csg'], default='csi_calorimeter',
+                    help='instanced_csg: only the timed 10M-photon instanced-CSG launches (for profiling that '
+                         'kernel); the default is the full bench on the calorimeter event')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one MI355X: this engine runs one process on one GPU and the '
@@ -273,6 +301,10 @@ def main():
     ensure_built()
     from oxr import abi, scenes, gensteps as G
     from oxr import engine as engine_mod
+
+    if args.workload == 'instanced_csg':
+        instanced_only(engine_mod, abi, scenes, args)
+        return
 
     # ---------------------------------------------------------------- headline
     name = 'csi_calorimeter'
