@@ -64,8 +64,11 @@ def test_random_scene_bitexact(engine_mod, oracle_lib, seed):
         assert np.array_equal(bits(e.photons()), bits(want['photon']))
         assert np.array_equal(e.seq(), want['seq'])
     prod = abi.default_config(max_bounce=32)
-    with engine_mod.Engine(g, p, prod) as e:      # production kernels, both forms
+    with engine_mod.Engine(g, p, prod) as e:      # production kernels, both forms, every traversal form
         e.set_gensteps(gs)
-        for form in (abi.OXR_FORM_PERSISTENT, abi.OXR_FORM_PLAIN):
-            e.launch(form)
-            assert np.array_equal(bits(e.photons()), bits(want['photon'])), form
+        for bvh in (0, 1):
+            for stack in (0, 1):
+                e.set_tuning(bvh_mode=bvh, stack_mode=stack)
+                for form in (abi.OXR_FORM_PERSISTENT, abi.OXR_FORM_PLAIN):
+                    e.launch(form)
+                    assert np.array_equal(bits(e.photons()), bits(want['photon'])), (bvh, stack, form)
