@@ -193,7 +193,9 @@ int Server::launch_step(Slot& s) {
     int form = cfg.form;
     int block = form == OXR_FORM_PERSISTENT ? e->tune.block_size : 256;
     oxd::Params P = e->params(s.buf, s.N, 0, 0, block);
-    P.seq = nullptr; P.record = nullptr; P.rec = nullptr; P.prd = nullptr;
+    // served requests return sphoton hits only: no per-photon debug or photonlite arrays,
+    // so the production kernels run whatever the engine's config asks of single launches
+    P.seq = nullptr; P.record = nullptr; P.rec = nullptr; P.prd = nullptr; P.lite = nullptr;
     P.segs = s.d_segs.p; P.nseg = int(s.segs.size());
     OXR_HIP(hipMemsetAsync(s.buf.counters.p, 0, 8 * sizeof(unsigned long long), s.stream));
     OXR_HIP(hipEventRecord(s.e0, s.stream));

@@ -24,9 +24,12 @@ def _events(s, rng, n, lo=500, hi=60000):
     return out
 
 
-def test_alone_equals_in_mix(engine_mod):
+@pytest.mark.parametrize('lite', [0, 1])
+def test_alone_equals_in_mix(engine_mod, lite):
+    """With photonlite on, single launches run the debug kernels (they write the lite
+    items) while batched steps run the production kernels: the hits still agree."""
     s = scenes.csi_calorimeter()
-    cfg = abi.default_config(**s.config)
+    cfg = abi.default_config(**s.config, photonlite=lite)
     rng = np.random.default_rng(12)
     evs = _events(s, rng, 12)
     with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
