@@ -57,3 +57,16 @@ def test_version_and_error_string(engine_mod):
     # a null engine is rejected on the host with a message, before any device call
     rc = L.oxr_engine_set_config(None, None)
     assert rc == -1 and b'null' in L.oxr_last_error()
+
+
+def test_python_constants_match_header():
+    """Every OXR_* enum value the header assigns equals the Python binding's constant of
+    the same name (gather kinds, forms, RNG schedules, status codes)."""
+    import os
+    import re
+    hdr = open(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'include', 'oxr.h')).read()
+    vals = {m.group(1): int(m.group(2), 0) for m in re.finditer(r'\b(OXR_[A-Z0-9_]+)\s*=\s*(-?\w+)', hdr)}
+    shared = [k for k in vals if hasattr(abi, k)]
+    assert {'OXR_GET_REC', 'OXR_GET_PRD', 'OXR_FORM_PERSISTENT', 'OXR_RNG_ALIGNED', 'OXR_OK'} <= set(shared)
+    for k in shared:
+        assert getattr(abi, k) == vals[k], k
