@@ -597,6 +597,7 @@ void oxr_engine_destroy(oxr_engine* eh) {
     if (e->e1) (void)hipEventDestroy(e->e1);
     if (e->e2) (void)hipEventDestroy(e->e2);
     if (e->eg) (void)hipEventDestroy(e->eg);
+    if (e->h_stage) (void)hipHostFree(e->h_stage);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -786,17 +787,42 @@ int oxr_event_get(oxr_engine* eh, int32_t what, void* dst, int64_t cap, int64_t*
     int64_t m = std::min(n, std::max<int64_t>(cap, 0));
     if (m > 0) {
         if (!dst) return fail(OXR_EINVAL, "null destination");
-        OXR_HIP(hipMemcpy(dst, src, size_t(m) * item, hipMemcpyDeviceToHost));
+        int rc = e->d2h(dst, src, size_t(m) * item);
+        if (rc) return rc;
     }
     return m < n ? fail(OXR_ECAPACITY, "destination smaller than count") : OXR_OK;
 }
 
-static int copy_out(const void* src, int64_t n, size_t item, void* dst, int64_t cap, int64_t* count) {
+// Device-to-host copy through the engine's pinned buffer, in chunks of at most 8 MB on the
+// engine stream (after the work that produced src): a copy straight into pageable memory
+// goes through the runtime's own staging at a few GB/s.
+int Engine::d2h(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kStage = size_t(8) << 20;
+    if (bytes == 0) return OXR_OK;
+    const size_t want = std::min(bytes, kStage);
+    if (h_stage_bytes < want) {
+        if (h_stage) (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        h_stage_bytes = 0;
+        OXR_HIP(hipHostMalloc(&h_stage, want, hipHostMallocDefault));
+        h_stage_bytes = want;
+    }
+    for (size_t off = 0; off < bytes; off += h_stage_bytes) {
+        const size_t k = std::min(h_stage_bytes, bytes - off);
+        OXR_HIP(hipMemcpyAsync(h_stage, static_cast<const char*>(src) + off, k, hipMemcpyDeviceToHost, stream));
+        OXR_HIP(hipStreamSynchronize(stream));
+        std::memcpy(static_cast<char*>(dst) + off, h_stage, k);
+    }
+    return OXR_OK;
+}
+
+static int copy_out(Engine& e, const void* src, int64_t n, size_t item, void* dst, int64_t cap, int64_t* count) {
     *count = n;
     int64_t m = std::min(n, std::max<int64_t>(cap, 0));
     if (m > 0) {
         if (!dst) return fail(OXR_EINVAL, "null destination");
-        OXR_HIP(hipMemcpy(dst, src, size_t(m) * item, hipMemcpyDeviceToHost));
+        int rc = e.d2h(dst, src, size_t(m) * item);
+        if (rc) return rc;
     }
     return m < n ? fail(OXR_ECAPACITY, "destination smaller than count") : OXR_OK;
 }
@@ -815,7 +841,7 @@ int oxr_event_get_merged(oxr_engine* eh, int32_t what, float time_window, void* 
     OXR_HIP(e->ev.merged.reserve(words * size_t(std::max<int64_t>(e->num_photon, 1))));
     long long nout = 0;
     OXR_HIP(oxl::merge(e->stream, lite, in, e->num_photon, e->cfg.hitmask, true, time_window, e->ev.merged.p, &nout));
-    return copy_out(e->ev.merged.p, nout, words * 4, dst, cap, count);
+    return copy_out(*e, e->ev.merged.p, nout, words * 4, dst, cap, count);
 }
 
 int oxr_merge_hits(oxr_engine* eh, int32_t what, float time_window, const void* in, int64_t n, void* dst, int64_t cap,
@@ -837,7 +863,7 @@ int oxr_merge_hits(oxr_engine* eh, int32_t what, float time_window, const void* 
     long long nout = 0;
     if ((he = oxl::merge(e->stream, lite, din.p, n, 0u, false, time_window, dout.p, &nout)) != hipSuccess)
         return done(hip_fail(he, "merge"));
-    return done(copy_out(dout.p, nout, words * 4, dst, cap, count));
+    return done(copy_out(*e, dout.p, nout, words * 4, dst, cap, count));
 }
 
 int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_index, int32_t form, void* hits,
@@ -869,7 +895,8 @@ int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_
         int64_t take = std::min(room, s.num_hit);
         if (take > 0 && hits) {
             std::lock_guard<std::mutex> lk(e->mu);
-            OXR_HIP(hipMemcpy(static_cast<char*>(hits) + size_t(total_hit) * 64, e->ev.hit.p, size_t(take) * 64, hipMemcpyDeviceToHost));
+            int rcc = e->d2h(static_cast<char*>(hits) + size_t(total_hit) * 64, e->ev.hit.p, size_t(take) * 64);
+            if (rcc) return rcc;
         }
         if (take < s.num_hit) overflow = true;
         total_hit += s.num_hit;

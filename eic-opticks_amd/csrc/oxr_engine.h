@@ -102,6 +102,10 @@ struct Engine {
     bool launched = false;
     bool hitlite_ready = false;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, eg = nullptr;
+    // pinned staging for gathers (grown on demand up to 8 MB); see d2h
+    void* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
+    int d2h(void* dst, const void* src, size_t bytes);
     std::mutex mu;   // serialises C-ABI calls on one engine
 
     oxd::Geom geom() const;
