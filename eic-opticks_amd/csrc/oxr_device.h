@@ -594,15 +594,20 @@ OXD bool leaf_phicut(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
     return valid;
 }
 
-// halfspace: plane p.n = w, inside p.n < w, exit at infinity signalled in isect.y
+// csg_intersect_leaf_halfspace.h:155-187: the halfspace p.n < w behind the plane p.n = w
+// (q0 = unit outward normal, w). The plane crossing t = -(o.n - w)/(d.n) (t_min for a ray
+// parallel to the plane) is a hit when beyond t_min. A miss whose origin lies inside
+// (o.n - w < -1e-9) signals the exit at infinity with isect.y = -0 for intersect_tree.
 OXD bool leaf_halfspace(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
-    f3 nn = mk(n.q0.x, n.q0.y, n.q0.z);
-    float w = n.q0.w;
-    float on = dot(o, nn), dn = dot(d, nn);
-    float t = (w - on) / dn;
-    bool valid = t > t_min && dn * (on - w) < 0.f;
+    const f3 nn = mk(n.q0.x, n.q0.y, n.q0.z);
+    const float w = n.q0.w;
+    const float on = dot(o, nn), dn = dot(d, nn);
+    const float on_w = on - w;
+    const bool inside = on_w < -1e-9f;
+    const float t = fabsf(dn) > 0.f ? -on_w / dn : t_min;
+    const bool valid = t > t_min;
     if (valid) { is.x = nn.x; is.y = nn.y; is.z = nn.z; is.w = t; }
-    else if (dn <= 0.f && on - w <= 0.f) { is.y = -is.y; }
+    else if (inside) { is.y = -0.f; }
     return valid;
 }
 

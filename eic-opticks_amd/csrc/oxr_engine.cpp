@@ -40,13 +40,13 @@ oxd::Geom Engine::geom() const {
     return g;
 }
 
-int Engine::stack_cap() const { return std::max(4, (use_flat() ? stack_need_f : stack_need) + 2); }
+int Engine::stack_cap(const oxr_tuning& t) const { return std::max(4, (use_flat(t) ? stack_need_f : stack_need) + 2); }
 
-int Engine::ntop_for(int block) const {
-    int want = std::max(0, tune.lds_top_nodes);
-    want = int(std::min<int64_t>(want, top_nodes()));
+int Engine::ntop_for(const oxr_tuning& t, int block) const {
+    int want = std::max(0, t.lds_top_nodes);
+    want = int(std::min<int64_t>(want, top_nodes(t)));
     size_t lim = 64 * 1024;
-    size_t stack = size_t(block) * stack_cap() * sizeof(int);
+    size_t stack = size_t(block) * stack_cap(t) * sizeof(int);
     if (stack >= lim) return 0;
     size_t room = (lim - stack) / sizeof(oxd::BNode);
     return int(std::min<size_t>(size_t(want), room));
@@ -56,16 +56,25 @@ int Engine::ntop_for(int block) const {
 // nodes, in a block's share of the CU's 160 KB of LDS at 4 waves per SIMD (16 waves, 1024
 // lanes per CU): more LDS per block would cost occupancy, which the latency-bound
 // traversal needs more than the culled pops.
-bool Engine::tn_stack_for(int block) const {
-    int want = int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), top_nodes()));
+bool Engine::tn_stack_for(const oxr_tuning& t, int block) const {
+    int want = int(std::min<int64_t>(std::max(0, t.lds_top_nodes), top_nodes(t)));
     size_t share = size_t(160 * 1024) * size_t(block) / 1024;
-    size_t need = size_t(want) * sizeof(oxd::BNode) + size_t(block) * stack_cap() * 8;
+    size_t need = size_t(want) * sizeof(oxd::BNode) + size_t(block) * stack_cap(t) * 8;
     return need <= share;
 }
 
+oxr_tuning Engine::effective(const oxr_tuning& t) const {
+    oxr_tuning r = t;
+    r.bvh_mode = use_flat(t) ? 1 : 0;
+    const bool tn = t.stack_mode == 1 && tn_stack_for(t, t.block_size);
+    r.stack_mode = tn ? 1 : 0;
+    r.lds_top_nodes = tn ? int(std::min<int64_t>(std::max(0, t.lds_top_nodes), top_nodes(t))) : ntop_for(t, t.block_size);
+    return r;
+}
+
 oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int block) const {
-    const bool tn = tune.stack_mode == 1 && tn_stack_for(block);
-    const int ntop = tn ? int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), top_nodes())) : ntop_for(block);
+    const bool tn = tune.stack_mode == 1 && tn_stack_for(tune, block);
+    const int ntop = tn ? int(std::min<int64_t>(std::max(0, tune.lds_top_nodes), top_nodes())) : ntop_for(tune, block);
     oxd::Params P{};
     P.g = geom();
     P.t = tables;
@@ -168,7 +177,7 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
                int form, bool count, oxr_launch_stats* st, const float* pre, int npre, const float4* input) {
     auto t0 = std::chrono::steady_clock::now();
     int block = form == OXR_FORM_PERSISTENT ? e.tune.block_size : 256;
-    if (block < 64 || block > 1024 || block % 64) return fail(OXR_EINVAL, "tuning block_size must be a multiple of 64 in [64,1024]");
+    if (block < 64 || block > 256 || block % 64) return fail(OXR_EINVAL, "tuning block_size must be a multiple of 64 in [64,256]");
     oxd::Params P = e.params(b, n, slot_off, evt_idx, block);
     P.pre = pre; P.npre = npre; P.input = input;
     size_t smem = oxl::simulate_smem(P, block);
@@ -334,6 +343,14 @@ extern "C" {
 
 const char* oxr_version(void) { return "oxr 0.1 gfx950"; }
 const char* oxr_last_error(void) { return g_err.c_str(); }
+
+// The simulate kernels are compiled for at most 256 lanes per workgroup
+// (__launch_bounds__(256, N)): a larger block would fail at launch.
+static bool tuning_ok(const oxr_tuning& t) {
+    return t.block_size >= 64 && t.block_size <= 256 && t.block_size % 64 == 0 && t.blocks_per_cu >= 1 &&
+           t.blocks_per_cu <= 64 && t.lds_top_nodes >= 0 && t.refill_batch >= 1 && t.refill_batch <= 64 &&
+           t.stack_mode >= 0 && t.stack_mode <= 1 && t.bvh_mode >= 0 && t.bvh_mode <= 1;
+}
 
 static bool config_ok(const oxr_config& c) {
     if (c.max_record < 0 || c.max_photon < 1 || c.max_rec < 0 || c.max_prd < 0) return false;
@@ -582,6 +599,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tables.icdf_width = ph->icdf ? ph->icdf_width : 0;
     e->tables.hd_factor = ph->icdf ? ph->hd_factor : 0;
     if ((he = hipDeviceSynchronize()) != hipSuccess) return bail(hip_fail(he, "upload sync"));
+    e->tune = e->effective(e->tune);
     *out = reinterpret_cast<oxr_engine*>(e);
     return OXR_OK;
 }
@@ -614,20 +632,27 @@ int oxr_engine_set_config(oxr_engine* eh, const oxr_config* c) {
 
 int oxr_engine_set_tuning(oxr_engine* eh, const oxr_tuning* t) {
     Engine* e = reinterpret_cast<Engine*>(eh);
-    if (!e || !t) return fail(OXR_EINVAL, "null argument");
-    if (t->block_size < 64 || t->block_size > 1024 || t->block_size % 64 || t->blocks_per_cu < 1 || t->blocks_per_cu > 64 ||
-        t->lds_top_nodes < 0 || t->refill_batch < 1 || t->stack_mode < 0 || t->stack_mode > 1 ||
-        t->bvh_mode < 0 || t->bvh_mode > 1)
-        return fail(OXR_EINVAL, "tuning out of range");
+    if (!t) return fail(OXR_EINVAL, "null argument");
+    if (!tuning_ok(*t)) return fail(OXR_EINVAL, "tuning out of range");
+    if (!e) return fail(OXR_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(e->mu);
-    e->tune = *t;
+    e->tune = e->effective(*t);
     return OXR_OK;
 }
 
 int oxr_engine_get_tuning(oxr_engine* eh, oxr_tuning* t) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !t) return fail(OXR_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
     *t = e->tune;
+    return OXR_OK;
+}
+
+int oxr_engine_get_config(oxr_engine* eh, oxr_config* c) {
+    Engine* e = reinterpret_cast<Engine*>(eh);
+    if (!e || !c) return fail(OXR_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);
+    *c = e->cfg;
     return OXR_OK;
 }
 
@@ -647,22 +672,28 @@ int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
     return OXR_OK;
 }
 
+// The engine's current event (QEvt role); callers hold e.mu.
+static int set_gensteps_locked(Engine& e, const float* gs, int64_t num_gs, int32_t event_index,
+                               uint64_t photon_slot_offset, int64_t* num_photon) {
+    int64_t n = 0;
+    int rc = upload_gensteps(e, e.ev, e.stream, gs, num_gs, &n);
+    if (rc) return rc;
+    e.h_gs.assign(gs, gs + 24 * num_gs);
+    e.num_gs = num_gs;
+    e.num_photon = n;
+    e.event_index = event_index;
+    e.slot_offset = photon_slot_offset;
+    e.launched = false;
+    *num_photon = n;
+    return OXR_OK;
+}
+
 int oxr_event_set_gensteps(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_index,
                            uint64_t photon_slot_offset, int64_t* num_photon) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !num_photon) return fail(OXR_EINVAL, "null argument");
     std::lock_guard<std::mutex> lk(e->mu);
-    int64_t n = 0;
-    int rc = upload_gensteps(*e, e->ev, e->stream, gs, num_gs, &n);
-    if (rc) return rc;
-    e->h_gs.assign(gs, gs + 24 * num_gs);
-    e->num_gs = num_gs;
-    e->num_photon = n;
-    e->event_index = event_index;
-    e->slot_offset = photon_slot_offset;
-    e->launched = false;
-    *num_photon = n;
-    return OXR_OK;
+    return set_gensteps_locked(*e, gs, num_gs, event_index, photon_slot_offset, num_photon);
 }
 
 int oxr_event_set_input_photons(oxr_engine* eh, const float* photons, int64_t n) {
@@ -706,24 +737,33 @@ static int check_sources(Engine& e, int64_t n, uint64_t slot_off) {
     return OXR_OK;
 }
 
+// One launch over the current event; callers hold e.mu. The event buffers are sized
+// again for the current config first: oxr_engine_set_config may have switched on or
+// enlarged a debug array (seq, record, rec, prd, photonlite) after the gensteps came.
+static int simulate_launch_locked(Engine& e, int32_t form, bool count, oxr_launch_stats* st) {
+    if (!count && e.num_gs == 0 && e.num_photon == 0) {
+        if (st) std::memset(st, 0, sizeof(*st));
+        e.launched = true;
+        e.num_hit = 0;
+        return OXR_OK;
+    }
+    int rc = check_sources(e, e.num_photon, e.slot_offset);
+    if (rc) return rc;
+    rc = ensure_event_capacity(e, e.ev, e.num_photon);
+    if (rc) return rc;
+    rc = run_launch(e, e.ev, e.stream, e.num_photon, int64_t(e.slot_offset), e.event_index, form, count, st,
+                    e.have_pre ? e.ev.pre.p : nullptr, e.have_pre ? e.npre : 0,
+                    e.have_input ? e.ev.input.p : nullptr);
+    if (rc == OXR_OK) e.launched = true;
+    return rc;
+}
+
 int oxr_simulate_launch(oxr_engine* eh, int32_t form, oxr_launch_stats* st) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return fail(OXR_EINVAL, "null engine");
     if (form != OXR_FORM_PLAIN && form != OXR_FORM_PERSISTENT) return fail(OXR_EINVAL, "unknown form");
     std::lock_guard<std::mutex> lk(e->mu);
-    if (e->num_gs == 0 && e->num_photon == 0) {
-        if (st) std::memset(st, 0, sizeof(*st));
-        e->launched = true;
-        e->num_hit = 0;
-        return OXR_OK;
-    }
-    int rc = check_sources(*e, e->num_photon, e->slot_offset);
-    if (rc) return rc;
-    rc = run_launch(*e, e->ev, e->stream, e->num_photon, int64_t(e->slot_offset), e->event_index, form, false, st,
-                    e->have_pre ? e->ev.pre.p : nullptr, e->have_pre ? e->npre : 0,
-                    e->have_input ? e->ev.input.p : nullptr);
-    if (rc == OXR_OK) e->launched = true;
-    return rc;
+    return simulate_launch_locked(*e, form, false, st);
 }
 
 // Counting launch (node visits / prim tests) on the current event: test and
@@ -732,13 +772,7 @@ int oxr_simulate_launch_counting(oxr_engine* eh, oxr_launch_stats* st) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e) return fail(OXR_EINVAL, "null engine");
     std::lock_guard<std::mutex> lk(e->mu);
-    int rc = check_sources(*e, e->num_photon, e->slot_offset);
-    if (rc) return rc;
-    rc = run_launch(*e, e->ev, e->stream, e->num_photon, int64_t(e->slot_offset), e->event_index, OXR_FORM_PLAIN, true, st,
-                    e->have_pre ? e->ev.pre.p : nullptr, e->have_pre ? e->npre : 0,
-                    e->have_input ? e->ev.input.p : nullptr);
-    if (rc == OXR_OK) e->launched = true;
-    return rc;
+    return simulate_launch_locked(*e, OXR_FORM_PLAIN, true, st);
 }
 
 int oxr_event_get(oxr_engine* eh, int32_t what, void* dst, int64_t cap, int64_t* count) {
@@ -876,6 +910,9 @@ int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_
     uint64_t slot = 0;
     int64_t g0 = 0;
     bool overflow = false;
+    // one lock over every slice's upload, launch and gather: no other call on this engine
+    // can replace the event between a launch and the copy of its hits
+    std::lock_guard<std::mutex> lk(e->mu);
     while (g0 < num_gs || (num_gs == 0 && g0 == 0)) {
         // genstep slice bounded by max_photon (SGenstep::GetGenstepSlices role)
         int64_t g1 = g0, np = 0;
@@ -886,15 +923,14 @@ int oxr_simulate(oxr_engine* eh, const float* gs, int64_t num_gs, int32_t event_
             np += k; g1++;
         }
         int64_t n = 0;
-        int rc = oxr_event_set_gensteps(eh, gs + 24 * g0, g1 - g0, event_index, slot, &n);
+        int rc = set_gensteps_locked(*e, gs + 24 * g0, g1 - g0, event_index, slot, &n);
         if (rc) return rc;
         oxr_launch_stats s{};
-        rc = oxr_simulate_launch(eh, form, &s);
+        rc = simulate_launch_locked(*e, form, false, &s);
         if (rc) return rc;
         int64_t room = std::max<int64_t>(0, hit_cap - total_hit);
         int64_t take = std::min(room, s.num_hit);
         if (take > 0 && hits) {
-            std::lock_guard<std::mutex> lk(e->mu);
             int rcc = e->d2h(static_cast<char*>(hits) + size_t(total_hit) * 64, e->ev.hit.p, size_t(take) * 64);
             if (rcc) return rcc;
         }
@@ -1005,18 +1041,26 @@ int oxr_intersect_prim(oxr_engine* eh, int32_t prim_index, const float* rays, in
 
 int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_results, int32_t* num_results) {
     Engine* e = reinterpret_cast<Engine*>(eh);
-    if (!e || repeat < 1) return fail(OXR_EINVAL, "bad argument");
+    if (!e || repeat < 1 || max_results < 0) return fail(OXR_EINVAL, "bad argument");
+    std::lock_guard<std::mutex> lk(e->mu);
     if (e->num_photon <= 0) return fail(OXR_ESTATE, "autotune needs an event with photons");
     oxr_tuning base = e->tune, best = e->tune;
     double best_ms = 1e300;
     int nres = 0;
-    auto measure = [&](const oxr_tuning& t, double& ms) -> int {
-        int rc = oxr_engine_set_tuning(eh, &t);
-        if (rc) return rc;
+    // every row records the tuning that ran (Engine::effective); a candidate that runs the
+    // same as one already measured is not launched again
+    std::vector<oxr_tuning> seen;
+    auto same = [](const oxr_tuning& a, const oxr_tuning& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; };
+    auto measure = [&](const oxr_tuning& t_in, double& ms) -> int {
+        if (!tuning_ok(t_in)) return fail(OXR_EINVAL, "tuning out of range");
+        const oxr_tuning t = e->effective(t_in);
+        for (const oxr_tuning& x : seen) if (same(x, t)) { ms = 1e300; return OXR_OK; }
+        seen.push_back(t);
+        e->tune = t;
         double tot = 0.0;
         for (int r = 0; r < repeat; r++) {
             oxr_launch_stats s{};
-            rc = oxr_simulate_launch(eh, OXR_FORM_PERSISTENT, &s);
+            int rc = simulate_launch_locked(*e, OXR_FORM_PERSISTENT, false, &s);
             if (rc) return rc;
             tot += s.kernel_ms;
         }
@@ -1035,7 +1079,7 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
     if (e->has_flat) {
         for (int bm : {0, 1}) {
             oxr_tuning t = base; t.bvh_mode = bm;
-            double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+            double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
         }
     }
     const oxr_tuning b0 = best;
@@ -1043,26 +1087,26 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         for (int bpc : {1, 2, 3, 4, 6, 8}) {
             oxr_tuning t = b0; t.block_size = bs; t.blocks_per_cu = bpc;
             if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, bs), bs) > 64 * 1024) continue;
-            double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+            double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
         }
     }
     oxr_tuning b1 = best;
     for (int nt : {0, 16, 64, 256, 512}) {
         oxr_tuning t = b1; t.lds_top_nodes = nt;
-        double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+        double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
     }
     oxr_tuning b2 = best;
     for (int rb : {1, 8, 16, 32, 48, 64}) {
         oxr_tuning t = b2; t.refill_batch = rb;
-        double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+        double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
     }
     oxr_tuning b3 = best;
     for (int sm : {0, 1}) {
         oxr_tuning t = b3; t.stack_mode = sm;
-        double ms; int rc = measure(t, ms); if (rc) { oxr_engine_set_tuning(eh, &base); return rc; }
+        double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
     }
-    oxr_engine_set_tuning(eh, &best);
-    if (num_results) *num_results = nres;
+    e->tune = best;
+    if (num_results) *num_results = std::min(nres, max_results);   // rows written to results
     return OXR_OK;
 }
 

@@ -76,8 +76,10 @@ struct Engine {
     int tlas_root_f = 0;
     int64_t tlas_nodes_f = 0;
     int stack_need_f = 0;
-    bool use_flat() const { return has_flat && tune.bvh_mode == 1; }
-    int64_t top_nodes() const { return use_flat() ? tlas_nodes_f : tlas_nodes; }
+    bool use_flat(const oxr_tuning& t) const { return has_flat && t.bvh_mode == 1; }
+    bool use_flat() const { return use_flat(tune); }
+    int64_t top_nodes(const oxr_tuning& t) const { return use_flat(t) ? tlas_nodes_f : tlas_nodes; }
+    int64_t top_nodes() const { return top_nodes(tune); }
     DBuf<uint4> optical;
     DBuf<float> icdf;
     int tlas_root = 0;
@@ -110,9 +112,13 @@ struct Engine {
 
     oxd::Geom geom() const;
     oxd::Params params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t evt_idx, int block) const;
-    int stack_cap() const;
-    int ntop_for(int block) const;
-    bool tn_stack_for(int block) const;
+    int stack_cap(const oxr_tuning& t) const;
+    int stack_cap() const { return stack_cap(tune); }
+    int ntop_for(const oxr_tuning& t, int block) const;
+    bool tn_stack_for(const oxr_tuning& t, int block) const;
+    // t as the launches of the persistent form run it: stack_mode 1 only where the
+    // distance-carrying stack fits (tn_stack_for), lds_top_nodes as many as are staged
+    oxr_tuning effective(const oxr_tuning& t) const;
 };
 
 // upload a gensteps slice into b, fill the seed array; returns photons in the slice

@@ -179,6 +179,9 @@ int Server::launch_step(Slot& s) {
     if (s.segs.empty()) return OXR_OK;
     s.t0 = std::chrono::steady_clock::now();
     s.N = cursor;
+    // the engine's config and tuning are read under its lock, so a concurrent
+    // oxr_engine_set_config/set_tuning takes effect between steps, never inside one
+    std::lock_guard<std::mutex> elk(e->mu);
     int rc = ensure_event_capacity(*e, s.buf, s.N);
     if (rc) return rc;
     int nb = oxl::compact_blocks(s.N);
@@ -337,7 +340,11 @@ int oxr_server_create(oxr_engine* eh, const oxr_server_config* cfg, oxr_server**
     for (auto& sl : s->slot) {
         if ((he = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "server stream"));
         if ((he = hipEventCreate(&sl.e0)) != hipSuccess || (he = hipEventCreate(&sl.e1)) != hipSuccess) return bail(hip_fail(he, "server events"));
-        int rc = ensure_event_capacity(*e, sl.buf, s->cfg.max_step_photons + 4096 * 64);
+        int rc;
+        {
+            std::lock_guard<std::mutex> elk(e->mu);
+            rc = ensure_event_capacity(*e, sl.buf, s->cfg.max_step_photons + 4096 * 64);
+        }
         if (rc) return bail(rc);
     }
     size_t fr = 0, tot = 0;

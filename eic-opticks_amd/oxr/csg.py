@@ -182,6 +182,16 @@ def phicut(phi0_pi, phi1_pi):
     return Node(CSG_PHICUT, (c0, s0, c1, s1, 0, 0), (-big, -big, -big, big, big, big))
 
 
+def halfspace(nx, ny, nz, w):
+    """Halfspace p.n < w behind the plane p.n = w, normal normalised (sysrap/sn.h:3417-3426,
+    CSG/csg_intersect_leaf_halfspace.h). Unbounded: its node box is zero as in
+    CSGNode::HalfSpace (CSG/CSGNode.cc:669-676), so a prim needs a bounded operand to
+    intersect it with."""
+    v = np.array([nx, ny, nz], dtype=np.float64)
+    v = v / np.linalg.norm(v)
+    return Node(CSG_HALFSPACE, (v[0], v[1], v[2], w, 0, 0), None)
+
+
 def complement(n):
     if n.subs is not None:
         raise ValueError('list nodes cannot be complemented')
@@ -273,7 +283,7 @@ def _aabb_of(n: Node):
             return None
         return np.concatenate([np.min([b[:3] for b in bs], axis=0), np.max([b[3:] for b in bs], axis=0)])
     if n.left is None:
-        if n.complement:
+        if n.complement or n.aabb is None:
             return None
         a = n.aabb
         if n.transform is not None:
@@ -418,9 +428,10 @@ class Foundry:
             else:
                 q[0:4] = n.param[0:4]
                 q[4:6] = n.param[4:6]
-                aabb = n.aabb if n.transform is None else transform_aabb(n.aabb, n.transform)
-                q[8:12] = aabb[0:4]
-                q[12:14] = aabb[4:6]
+                if n.aabb is not None:      # unbounded leaves (halfspace) keep a zero box
+                    aabb = n.aabb if n.transform is None else transform_aabb(n.aabb, n.transform)
+                    q[8:12] = aabb[0:4]
+                    q[12:14] = aabb[4:6]
                 u[14] = n.typecode
                 tidx = 0
                 if n.transform is not None and not np.allclose(n.transform, np.eye(4)):

@@ -50,6 +50,7 @@ def lib():
             'oxr_engine_set_config': (i32, [vp, P(abi.Config)]),
             'oxr_engine_set_tuning': (i32, [vp, P(abi.Tuning)]),
             'oxr_engine_get_tuning': (i32, [vp, P(abi.Tuning)]),
+            'oxr_engine_get_config': (i32, [vp, P(abi.Config)]),
             'oxr_engine_query': (i32, [vp, P(abi.EngineInfo)]),
             'oxr_event_set_gensteps': (i32, [vp, vp, i64, i32, u64, P(i64)]),
             'oxr_event_set_input_photons': (i32, [vp, vp, i64]),

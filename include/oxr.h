@@ -205,8 +205,14 @@ int  oxr_engine_create(const oxr_geometry* geo, const oxr_physics* phys,
                        const oxr_config* cfg, oxr_engine** out);
 void oxr_engine_destroy(oxr_engine* e);
 int  oxr_engine_set_config(oxr_engine* e, const oxr_config* cfg);
+/* The persistent form's launch settings. Stored (and returned by get_tuning) as the
+ * launches run them: stack_mode 1 falls back to 0 where the distance-carrying stack and
+ * the staged top nodes do not fit a workgroup's LDS share, and lds_top_nodes is clipped to
+ * what is staged. block_size is at most 256 (the kernels' launch bound). */
 int  oxr_engine_set_tuning(oxr_engine* e, const oxr_tuning* t);
 int  oxr_engine_get_tuning(oxr_engine* e, oxr_tuning* t);
+/* The current stepping/event config (the SEventConfig getters, sysrap/SEventConfig.hh). */
+int  oxr_engine_get_config(oxr_engine* e, oxr_config* cfg);
 int  oxr_engine_query(oxr_engine* e, oxr_engine_info* info);
 /* Fill *cfg with the reference defaults (SEventConfig.cc:45-106). */
 void oxr_config_defaults(oxr_config* cfg);
@@ -310,7 +316,10 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
 /* ---- autotuner ----------------------------------------------------------
  * Times the persistent form on the current event over a grid of oxr_tuning values
  * (repeat times each) and applies the fastest. results (max_results,7) f64 =
- * block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode, kernel_ms. */
+ * block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode, kernel_ms,
+ * one row per distinct effective tuning launched (see oxr_engine_set_tuning);
+ * *num_results = rows written (at most max_results). On an error the tuning in force
+ * before the call is restored. No reference counterpart. */
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 

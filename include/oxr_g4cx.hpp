@@ -61,6 +61,7 @@ struct quad6 {
     float f[24];
     uint32_t& u(int i) { return *reinterpret_cast<uint32_t*>(&f[i]); }
     int32_t& i(int k) { return *reinterpret_cast<int32_t*>(&f[k]); }
+    uint32_t numphoton() const { return reinterpret_cast<const uint32_t*>(f)[3]; }
 };
 static_assert(sizeof(quad6) == 96, "quad6 is 96 bytes");
 
@@ -231,9 +232,8 @@ public:
         if (!engine_) throw std::runtime_error("G4CXOpticks::simulate before SetGeometry");
         SEvt* ev = SEvt::Get_EGPU();
         const int64_t n = ev->GetNumPhotonCollected(0);
-        if (!ev->input_.empty())
-            check(oxr_event_set_input_photons(engine_, reinterpret_cast<const float*>(ev->input_.data()),
-                                               int64_t(ev->input_.size())), "set input photons");
+        check(oxr_event_set_input_photons(engine_, ev->input_.empty() ? nullptr : reinterpret_cast<const float*>(ev->input_.data()),
+                                           int64_t(ev->input_.size())), "set input photons");
         std::vector<sphoton> hits(size_t(n > 0 ? n : 0));
         int64_t num_hit = 0;
         oxr_launch_stats st{};
@@ -336,24 +336,48 @@ public:
     double launch() { return mode_ == SIMTRACE ? simtrace_launch() : simulate_launch(); }
     const char* desc() const { return desc_.c_str(); }
 
-    /* high level: the SEvt's gensteps in, hits (or simtrace items) out */
+    /* high level: the SEvt's gensteps in, hits (or simtrace items) out. Events larger
+     * than the engine's max_photon run as several launches over genstep slices, each with
+     * its photon slot offset, hits gathered after each (QSim::simulate, QSim.cc:455-526);
+     * returns the summed launch seconds. */
     double simulate(int eventID, bool reset_ = false) {
         SEvt* ev = SEvt::Get_EGPU();
         event_ = eventID;
-        if (!ev->input_.empty())
-            check(oxr_event_set_input_photons(engine_, reinterpret_cast<const float*>(ev->input_.data()),
-                                               int64_t(ev->input_.size())), "set input photons");
-        int64_t n = 0;
-        check(oxr_event_set_gensteps(engine_, reinterpret_cast<const float*>(ev->gensteps_.data()),
-                                     int64_t(ev->gensteps_.size()), eventID, 0, &n), "set gensteps");
-        const double s = simulate_launch();
-        int64_t nh = 0;
-        int rc = oxr_event_get(engine_, OXR_GET_HIT, nullptr, 0, &nh);
-        if (rc != OXR_OK && rc != OXR_ECAPACITY) check(rc, "hit count");
-        ev->hits_.assign(size_t(nh), sphoton{});
-        if (nh > 0) check(oxr_event_get(engine_, OXR_GET_HIT, ev->hits_.data(), nh, &nh), "gather hits");
+        // input photons belong to this event only: an event without them clears the
+        // engine's copy, so INPUT_PHOTON gensteps cannot read an earlier event's photons
+        check(oxr_event_set_input_photons(engine_, ev->input_.empty() ? nullptr : reinterpret_cast<const float*>(ev->input_.data()),
+                                           int64_t(ev->input_.size())), "set input photons");
+        oxr_config cfg{};
+        check(oxr_engine_get_config(engine_, &cfg), "get config");
+        const float* gs = reinterpret_cast<const float*>(ev->gensteps_.data());
+        const int64_t ngs = int64_t(ev->gensteps_.size());
+        ev->hits_.clear();
+        double seconds = 0.0;
+        uint64_t slot = 0;
+        int64_t g0 = 0;
+        do {
+            int64_t g1 = g0, np = 0;
+            while (g1 < ngs) {
+                const int64_t k = int64_t(ev->gensteps_[size_t(g1)].numphoton());
+                if (k > cfg.max_photon) throw std::runtime_error("oxr: a single genstep exceeds max_photon");
+                if (np + k > cfg.max_photon) break;
+                np += k;
+                g1++;
+            }
+            int64_t n = 0;
+            check(oxr_event_set_gensteps(engine_, gs + 24 * g0, g1 - g0, eventID, slot, &n), "set gensteps");
+            seconds += simulate_launch();
+            int64_t nh = 0;
+            int rc = oxr_event_get(engine_, OXR_GET_HIT, nullptr, 0, &nh);
+            if (rc != OXR_OK && rc != OXR_ECAPACITY) check(rc, "hit count");
+            const size_t at = ev->hits_.size();
+            ev->hits_.resize(at + size_t(nh));
+            if (nh > 0) check(oxr_event_get(engine_, OXR_GET_HIT, ev->hits_.data() + at, nh, &nh), "gather hits");
+            slot += uint64_t(n);
+            g0 = g1;
+        } while (g0 < ngs);
         if (reset_) reset(eventID);
-        return s;
+        return seconds;
     }
     double simtrace(int eventID) {
         SEvt* ev = SEvt::Get_EGPU();

@@ -569,16 +569,20 @@ void leaf_hyperboloid(bool& valid, Isect<T>& is, const Node& n, T t_min, V3<T> o
     }
 }
 
-// csg_intersect_leaf_halfspace.h : p.n = w plane, inside is p.n < w; unbounded exit
+// csg_intersect_leaf_halfspace.h:155-187: halfspace p.n < w behind the plane p.n = w, q0 =
+// (unit outward normal, w). t = -(o.n - w)/(d.n), or t_min for a ray parallel to the plane;
+// a miss from inside (o.n - w < -1e-9) signals the exit at infinity with isect.y = -0.
 template <typename T>
 void leaf_halfspace(bool& valid, Isect<T>& is, const Node& n, T t_min, V3<T> o, V3<T> d) {
     V3<T> nn = v3<T>(n.f[0], n.f[1], n.f[2]);
     T w = n.f[3];
     T on = dot(o, nn), dn = dot(d, nn);
-    T t_cand = (w - on) / dn;
-    valid = t_cand > t_min && dn * (on - w) < T(0);
-    if (valid) { is.x = nn.x; is.y = nn.y; is.z = nn.z; is.w = t_cand; }
-    else if (dn <= T(0) && on - w <= T(0)) { is.y = -is.y; }   // exit at infinity
+    T on_w = on - w;
+    bool inside = on_w < T(-1e-9f);
+    T t = std::fabs(dn) > T(0) ? -on_w / dn : t_min;
+    valid = t > t_min;
+    if (valid) { is.x = nn.x; is.y = nn.y; is.z = nn.z; is.w = t; }
+    else if (inside) { is.y = -T(0); }
 }
 
 // qat4::right_multiply (sysrap/sqat4.h:45-51) for an itra/instance matrix stored as float[16]

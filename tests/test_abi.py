@@ -70,3 +70,20 @@ def test_python_constants_match_header():
     assert {'OXR_GET_REC', 'OXR_GET_PRD', 'OXR_FORM_PERSISTENT', 'OXR_RNG_ALIGNED', 'OXR_OK'} <= set(shared)
     for k in shared:
         assert getattr(abi, k) == vals[k], k
+
+
+def test_tuning_range_checked_on_the_host(engine_mod):
+    """oxr_engine_set_tuning validates the tuning before touching the engine: a block of
+    512 lanes (beyond the kernels' 256-lane launch bound), a refill batch beyond a wave or
+    a stack mode outside 0/1 is rejected as out of range; an in-range tuning reaches the
+    null-engine check."""
+    L = engine_mod.lib()
+    good = dict(block_size=256, blocks_per_cu=4, lds_top_nodes=64, refill_batch=8, stack_mode=1, bvh_mode=0)
+    for bad in (dict(block_size=512), dict(block_size=1024), dict(block_size=96), dict(refill_batch=65),
+                dict(stack_mode=2), dict(blocks_per_cu=0)):
+        t = abi.Tuning(**dict(good, **bad))
+        assert L.oxr_engine_set_tuning(None, C.byref(t)) == abi.OXR_EINVAL
+        assert b'out of range' in L.oxr_last_error(), bad
+    t = abi.Tuning(**good)
+    assert L.oxr_engine_set_tuning(None, C.byref(t)) == abi.OXR_EINVAL
+    assert b'null' in L.oxr_last_error()

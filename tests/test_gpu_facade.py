@@ -92,3 +92,64 @@ int main(int argc, char** argv) {
     out = dict(l.split(' ', 1) for l in r.stdout.strip().splitlines())
     assert out['hits'] == '10 seconds_positive 1 gensteps_after_reset 0'
     assert out['desc'].startswith('oxr') and out['render_throws'] == '1'
+
+
+SLICED = r'''
+#include "oxr_g4cx.hpp"
+#include <cstdio>
+int main(int argc, char** argv) {
+    oxr::G4CXOpticks* gx = oxr::G4CXOpticks::SetGeometry(argv[1]);
+    oxr::Simulator<> sim(gx->engine(), OXR_FORM_PLAIN);
+    oxr::SEvt* ev = oxr::SEvt::Get_EGPU();
+    std::vector<oxr::sphoton> ten = oxr::load_photons_txt(argv[2]), all;
+    for (int k = 0; k < 300; k++) all.insert(all.end(), ten.begin(), ten.end());
+    ev->input_ = all;
+    for (int k = 0; k < 3; k++) {            /* three INPUT_PHOTON gensteps of 1000 photons */
+        oxr::quad6 g{};
+        g.i(0) = 19; g.i(3) = 1000;
+        ev->addGenstep(g);
+    }
+    sim.simulate(0, true);
+    unsigned long long h = 1469598103934665603ull;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(ev->hits_.data());
+    for (size_t i = 0; i < ev->hits_.size() * sizeof(oxr::sphoton); i++) { h ^= b[i]; h *= 1099511628211ull; }
+    std::printf("hits %u fnv %llx\n", ev->GetNumHit(0), h);
+    /* the next event has an INPUT_PHOTON genstep but no input photons of its own */
+    oxr::quad6 g{};
+    g.i(0) = 19; g.i(3) = 10;
+    ev->addGenstep(g);
+    int refused = 0;
+    try { sim.simulate(1, true); } catch (const std::exception&) { refused = 1; }
+    std::printf("stale_input_refused %d\n", refused);
+    oxr::G4CXOpticks::Finalize();
+    return 0;
+}
+'''
+
+
+def test_simulator_slices_large_events_and_clears_input(raindrop_folder, tmp_path):
+    """oxr::Simulator::simulate on an event above max_photon (OPTICKS_MAX_SLOT=1500, 3000
+    input photons in three gensteps) runs genstep slices with photon slot offsets and
+    gathers every slice's hits: the same hits, in the same order, as one launch. An event
+    that brings an INPUT_PHOTON genstep without input photons is refused instead of
+    reading the previous event's photons."""
+    src = tmp_path / 'sliced.cpp'
+    src.write_text(SLICED)
+    exe = tmp_path / 'sliced'
+    lib = os.path.dirname(EXE)
+    subprocess.check_call(['g++', '-std=c++17', '-Wall', '-I', os.path.join(ROOT_DIR, 'include'), str(src), '-o', str(exe),
+                           '-L', lib, '-loxr', f'-Wl,-rpath,{lib}'])
+    outs = {}
+    for slot in ('1500', '0'):
+        env = dict(os.environ)
+        env.pop('OPTICKS_MAX_SLOT', None)
+        if slot != '0':
+            env['OPTICKS_MAX_SLOT'] = slot
+        print('sliced facade run, OPTICKS_MAX_SLOT', slot, flush=True)
+        r = subprocess.run([str(exe), raindrop_folder, os.path.join(GOLDEN, 'raindrop_photons.txt')], capture_output=True,
+                           text=True, timeout=120, env=env)
+        assert r.returncode == 0, r.stderr
+        outs[slot] = dict(l.split(' ', 1) for l in r.stdout.strip().splitlines())
+    assert outs['1500']['hits'].startswith('3000 ')
+    assert outs['1500']['hits'] == outs['0']['hits']
+    assert outs['1500']['stale_input_refused'] == '1' and outs['0']['stale_input_refused'] == '1'

@@ -5,7 +5,7 @@ import pytest
 
 from oxr import abi, gensteps as G, scenes
 from oxr.csg import (Foundry, box3, sphere, zsphere, cylinder, cone, convexpolyhedron, phicut, scale, translate,
-                     rotate_z_to, contiguous, discontiguous, overlap, hyperboloid)
+                     rotate_z_to, contiguous, discontiguous, overlap, hyperboloid, halfspace)
 
 pytestmark = pytest.mark.gpu
 
@@ -223,6 +223,9 @@ def zoo():
         scenes.pmt_like_solid(),
         (sphere(80.0, 30.0, 0.0, 0.0) & box3(120.0)) - cylinder(20.0, -200.0, 200.0),
         (box3(100.0) | sphere(70.0, 0.0, 0.0, 60.0)) - (cylinder(30.0, -100.0, 100.0) | box3(20.0, 200.0, 20.0)),
+        halfspace(1.0, 1.0, 0.0, 20.0) & box3(150.0),
+        box3(150.0) - halfspace(0.3, -0.5, 0.8, 15.0),
+        halfspace(0.0, 1.0, 0.0, 10.0) & cylinder(60.0, -80.0, 80.0),
     ]
     for t in prims:
         f.add_prim(t, b)
@@ -391,3 +394,26 @@ def test_list_node_validation(engine_mod):
     a['node'].view(np.uint32).reshape(-1, 16)[0, 1] = 7        # subOffset past the prim's nodes
     with pytest.raises(engine_mod.OxrError, match='subs outside'):
         engine_mod.Engine(a, f.physics(), abi.default_config())
+
+
+def test_debug_arrays_enabled_after_gensteps(engine_mod, oracle_lib):
+    """seqhis and records switched on (and enlarged) by set_config after the gensteps of a
+    larger event were uploaded: the launch sizes the event buffers for the config in force,
+    and the arrays equal the oracle's."""
+    s = scene('nested_boxes')
+    small = s.gensteps(scale_photons=0.001)
+    big = s.gensteps(scale_photons=0.3)
+    with engine_mod.Engine(s.geometry, s.physics, abi.default_config(**s.config, seq_enabled=1, max_record=2)) as e:
+        e.set_gensteps(small)
+        e.launch()
+        e.set_config(seq_enabled=0, max_record=0)
+        n = e.set_gensteps(big)
+        e.set_config(seq_enabled=1, max_record=6)
+        print('debug arrays after gensteps:', n, 'photons', flush=True)
+        e.launch()
+        ph, seq, rec = e.photons(), e.seq(), e.get(abi.OXR_GET_RECORD)
+    cfg = abi.default_config(**s.config, seq_enabled=1, max_record=6)
+    r = run_oracle(oracle_lib, s, big, cfg)
+    assert np.array_equal(bits(ph), bits(r['photon']))
+    assert np.array_equal(seq, r['seq'])
+    assert np.array_equal(bits(rec), bits(r['record']))
