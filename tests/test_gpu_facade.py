@@ -150,6 +150,23 @@ def test_simulator_slices_large_events_and_clears_input(raindrop_folder, tmp_pat
                            text=True, timeout=120, env=env)
         assert r.returncode == 0, r.stderr
         outs[slot] = dict(l.split(' ', 1) for l in r.stdout.strip().splitlines())
-    assert outs['1500']['hits'].startswith('3000 ')
+    # the oracle on the same event: the ten photons repeated 300 times behind three
+    # INPUT_PHOTON gensteps of 1000, default config, hits in photon order
+    import oracle as O
+    from oxr import abi, sources
+    g = gdml.load_gdml(geom('opticks_raindrop'))
+    inp = np.tile(sources.load_photons_txt(os.path.join(GOLDEN, 'raindrop_photons.txt')), (300, 1, 1))
+    gs = np.zeros((3, 6, 4), dtype=np.float32)
+    gs.view(np.int32)[:, 0, 0] = 19
+    gs.view(np.int32)[:, 0, 3] = 1000
+    cfg = abi.default_config()
+    ref = O.Oracle(g.geometry, g.physics, cfg).simulate(gs, input_photons=inp, seq=False)
+    u = ref['photon'].view(np.uint32)
+    hits = ref['photon'][(u[:, 3, 3] & cfg.hitmask) == cfg.hitmask]
+    h = 1469598103934665603
+    for b in hits.tobytes():
+        h = ((h ^ b) * 1099511628211) & 0xffffffffffffffff
+    assert len(hits) > 2900
+    assert outs['0']['hits'] == f'{len(hits)} fnv {h:x}'
     assert outs['1500']['hits'] == outs['0']['hits']
     assert outs['1500']['stale_input_refused'] == '1' and outs['0']['stale_input_refused'] == '1'
