@@ -1,14 +1,18 @@
 """Benchmark: photons/s and bounce-steps/s on one MI355X.
 
 Headline step: one simulate launch (generate + propagate + hit compaction) of the
-crystal-calorimeter workload (8x8 CsI crystals with SiPMs, 2000 scintillation + 500
-Cerenkov gensteps, ~2.08M photons) with gensteps resident in HBM, persistent form with
-the committed autotuned launch settings. Also reported: the plain form, the roofline of
-the simulate kernel, a CPU baseline (the CPU restatement on the host cores), the serving
-form's p50/p99 event latency against sequential per-event launches, and served vs plain
-photons/s on the instanced-CSG geometry.
+instanced-CSG workload (water sphere r=19.5 m in an acrylic shell, 20,000 instances of a
+7-node CSG solid, 10M torch photons from 1,000 interior points, max_bounce 64) with
+gensteps resident in HBM, persistent lane-refill form with the committed autotuned
+launch settings, timed over --steps launches. Also reported: the plain form, the
+roofline of that launch, a CPU baseline (the CPU restatement on every host core), the
+calorimeter, nested-box and thin-shell workloads, the serving form on its 256-event
+stream (p50/p99 event latency against sequential per-event launches) and served vs
+per-event photons/s on the instanced-CSG geometry.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+python bench.py --workload {instanced_csg,csi_calorimeter,thin_shells,nested_boxes} --only
+    (the timed launches of one workload alone, for kernel traces and counters)
 """
 from __future__ import annotations
 
@@ -97,25 +101,43 @@ def timed_launches(e, form, steps, warmup):
     return dt, st, float(np.mean(ks))
 
 
-def cpu_baseline(scene, gs, cfg, budget_s=12.0):
-    """The CPU restatement (oracle) of the same workload on the host cores: a bounded
-    sample of the event's photons, photons/s over the sample."""
+# The GPU host allots 16 cores to a one-GPU job, whatever os.cpu_count() shows there.
+HOST_CORE_SHARE = 16
+
+
+def host_threads():
+    """The host cores this process may use: its affinity mask, within the host's share."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, HOST_CORE_SHARE))
+
+
+def cpu_baseline(scene, gs, cfg, threads, budget_s=12.0, inp=None):
+    """The CPU restatement (oracle) of the same workload on `threads` host threads: a
+    bounded sample of the event's first photons, photons/s over the sample."""
     import oracle as O
-    threads = min(16, os.cpu_count() or 1)
     o = O.Oracle(scene.geometry, scene.physics, cfg)
     o.set_cull(True)
     n_total = int(np.asarray(gs).view(np.uint32)[:, 0, 3].astype(np.int64).sum())
-    probe = min(4000, n_total)
+    probe = min(max(2000, 64 * threads), n_total)
     t0 = time.perf_counter()
-    o.simulate(gs, nthreads=threads, begin=0, end=probe, seq=False)
+    o.simulate(gs, nthreads=threads, begin=0, end=probe, seq=False, input_photons=inp)
     rate = probe / max(time.perf_counter() - t0, 1e-6)
     sample = int(min(n_total, max(probe, rate * budget_s)))
     t0 = time.perf_counter()
-    r = o.simulate(gs, nthreads=threads, begin=0, end=sample, seq=False)
+    r = o.simulate(gs, nthreads=threads, begin=0, end=sample, seq=False, input_photons=inp)
     dt = time.perf_counter() - t0
-    return dict(value=sample / dt, unit='photons/s', cores=threads, kind='port',
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    return dict(value=round(sample / dt, 1), unit='photons/s', cores=threads, nproc=os.cpu_count(),
+                affinity_cores=affinity, core_share=HOST_CORE_SHARE, kind='port',
                 sample=f'first {sample} of {n_total} photons of the same event, {threads} std::threads, '
-                       f'{dt:.1f} s', bounce_steps_per_s=float(r['bounce'].sum()) / dt)
+                       f'{dt:.1f} s', bounce_steps_per_s=round(float(r['bounce'].sum()) / dt, 1),
+                photons_per_s_per_core=round(sample / dt / threads, 1))
 
 
 def history_chi2_leg(engine_mod, scene, gs, cfg, sample, threads, tune=None, inp=None, begin=0):
@@ -193,19 +215,6 @@ def precision_leg(engine_mod, threads, sample=5000):
     return rows, (s, gs, inp, ng)
 
 
-def serving_events(rng, n_events, median, sigma, lo, hi, ext):
-    counts = np.clip(np.exp(np.log(median) + sigma * rng.normal(size=n_events)), lo, hi).astype(np.int64)
-    from oxr import gensteps as G
-    evs = []
-    for c in counts:
-        ngs = int(max(1, min(64, c // 20000)))
-        pos = rng.uniform(-ext, ext, size=(ngs, 3))
-        per = np.full(ngs, c // ngs)
-        per[0] += c - per.sum()
-        evs.append(G.torch(per, pos=pos, wavelength=420.0, type=G.T_POINT_ISOTROPIC))
-    return evs, counts
-
-
 def run_server(engine_mod, e, evs, arrivals, step_photons):
     srv = engine_mod.Server(e, max_step_photons=step_photons)
     res = {}
@@ -256,29 +265,95 @@ def run_sequential(e, evs, arrivals):
     return lat, time.perf_counter() - t0, photons, busy
 
 
-def instanced_only(engine_mod, abi, scenes, args):
-    """The instanced-CSG event alone (persistent then plain launches, committed tuning),
-    so that a kernel trace of this command holds only its 10M-photon launches."""
-    s3 = scenes.instanced_csg()
-    cfg3 = abi.default_config(**s3.config)
-    with engine_mod.Engine(s3.geometry, s3.physics, cfg3) as e3:
-        t3 = load_tuning('instanced_csg')
-        if t3:
-            e3.set_tuning(**t3)
-        n3 = e3.set_gensteps(s3.gensteps())
-        dt, st, k = timed_launches(e3, abi.OXR_FORM_PERSISTENT, args.steps, args.warmup)
-        dtp, _, kp = timed_launches(e3, abi.OXR_FORM_PLAIN, args.steps, args.warmup)
-        tuning = e3.tuning()
-    out = {'metric': 'photons/s on 1 MI355X (instanced-CSG event)', 'value': round(args.steps * n3 / dt, 1),
-           'unit': 'photons/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-           'ms_per_step': round(1e3 * dt / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
-           'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
-           'config': {'workload': 'instanced_csg: water sphere r=19.5 m, acrylic shell, 20000 instanced 7-node '
-                                  'CSG solids, 10M torch photons', 'photons': n3, 'form': 'persistent',
-                      'tuning': tuning},
-           'kernel_ms': round(k, 3), 'bounce_steps': int(st['bounce_steps']),
-           'plain_form': {'photons_per_s': round(args.steps * n3 / dtp, 1), 'kernel_ms': round(kp, 3)}}
-    print(json.dumps(out), flush=True)
+WORKLOADS = {
+    'instanced_csg': 'instanced_csg: water sphere r=19.5 m in a 120 mm acrylic shell, 20000 instanced 7-node CSG '
+                     'solids (zsphere|cylinder minus inner zsphere) at r=19.4 m, 10M torch photons at 430 nm from '
+                     '1000 interior points, max_bounce 64',
+    'csi_calorimeter': 'csi_calorimeter: 8x8 CsI 20x20x200 mm crystals + 6x6 mm SiPMs, 2000 scintillation + 500 '
+                       'Cerenkov gensteps, Poisson 1000/150 photons, max_bounce 32',
+    'nested_boxes': 'nested_boxes: 240/220/200/100 mm vacuum/lead/air/water cubes, 100k torch photons at 420 nm '
+                    'from the water centre, max_bounce 32',
+    'thin_shells': 'thin_shells: 50 x 1 mm shells at 1-20 m (n 1.50/1.33) + 5000 instanced zsphere|cylinder, '
+                   '5M torch photons at 400 nm from the origin, max_bounce 64',
+}
+
+
+def roofline_block(e, name, n, n_gs, n_hit, kms, generate_ms):
+    """Algorithmic bytes of one launch (counting launch over the same photons) over the
+    measured mean launch time, against the 8 TB/s HBM peak; beside it the HBM traffic and
+    the vector-issue counters rocprofv3 measured for this workload (committed summary)."""
+    cnt = e.launch(counting=True)
+    ab = algorithmic_bytes(cnt, n, n_gs, n_hit)
+    achieved = ab / (kms / 1e3) / 1e9
+    prof = read_profile(name)
+    traffic = prof.get('traffic')
+    return dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel='k_generate + k_simulate_persist (one simulate launch)',
+                generate_ms=round(generate_ms, 4), propagate_ms=round(kms - generate_ms, 4),
+                algorithmic_bytes_per_launch=int(ab),
+                node_visits=int(cnt['node_visits']), prim_tests=int(cnt['prim_tests']),
+                csg_node_bytes=int(cnt['csg_node_bytes']), instance_visits=int(cnt['instance_visits']),
+                measured_limiter='vector ALU issue under lane divergence' if prof else None,
+                hbm_measured_frac=(round(traffic / (kms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
+                valu_busy_pct=prof.get('valu_busy'), valu_lane_util_pct=prof.get('valu_util'),
+                counters_source=(f'profiles/pmc_traffic.txt ({name}): rocprofv3 --pmc on bench.py --workload {name} '
+                                 '--only, read from the committed summary, not measured in this run') if prof else None)
+
+
+def setup(engine_mod, abi, scenes, name):
+    from oxr import gensteps as G
+    s = scenes.SCENES[name]()
+    cfg = abi.default_config(**s.config)
+    e = engine_mod.Engine(s.geometry, s.physics, cfg)
+    tune = load_tuning(name)
+    if tune:
+        e.set_tuning(**tune)
+    inp = None
+    if name == 'thin_shells':
+        n_total = 5_000_000
+        ng = n_total // 10
+        inp = scenes.thin_shell_grazing_photons(s, ng)
+        gs = np.concatenate([G.input_photon(ng), s.gensteps(scale_photons=(n_total - ng) / 5e6)])
+        e.set_input_photons(inp)
+    else:
+        gs = s.gensteps()
+    n = e.set_gensteps(gs)
+    return s, cfg, e, gs, n, tune, inp
+
+
+def measure(engine_mod, abi, scenes, name, steps, warmup, roofline=True):
+    """Persistent and plain forms of one workload, timed over `steps` launches each."""
+    s, cfg, e, gs, n, tune, inp = setup(engine_mod, abi, scenes, name)
+    dt, st, kms = timed_launches(e, abi.OXR_FORM_PERSISTENT, steps, warmup)
+    dtp, stp, kmsp = timed_launches(e, abi.OXR_FORM_PLAIN, steps, warmup)
+    out = dict(workload=WORKLOADS[name], photons=n, gensteps=len(gs),
+               photons_per_s=round(steps * n / dt, 1),
+               bounce_steps_per_s=round(steps * st['bounce_steps'] / dt, 1),
+               ms_per_step=round(1e3 * dt / steps, 3), kernel_ms=round(kms, 3),
+               hits=int(st['num_hit']), bounce_steps=int(st['bounce_steps']),
+               plain_form=dict(photons_per_s=round(steps * n / dtp, 1), kernel_ms=round(kmsp, 3),
+                               bounce_steps_per_s=round(steps * stp['bounce_steps'] / dtp, 1)),
+               persistent_vs_plain=round(dtp / dt, 3), tuning=e.tuning(), max_bounce=cfg.max_bounce)
+    log(f"[bench] {name}: persistent {out['photons_per_s'] / 1e6:.1f} M photons/s "
+        f"({kms:.2f} ms), plain {out['plain_form']['photons_per_s'] / 1e6:.1f} M photons/s ({kmsp:.2f} ms)")
+    if roofline:
+        e.set_gensteps(gs)
+        out['roofline'] = roofline_block(e, name, n, len(gs), st['num_hit'], kms, st['generate_ms'])
+    e.close()
+    return out, (s, cfg, gs, tune, inp)
+
+
+def only(engine_mod, abi, scenes, args):
+    """The timed launches of one workload alone (persistent then plain), so that a kernel
+    trace or counter run of this command holds only that workload's launches."""
+    out, _ = measure(engine_mod, abi, scenes, args.workload, args.steps, args.warmup, roofline=False)
+    print(json.dumps(dict(metric=f'photons/s on 1 MI355X ({args.workload})', value=out['photons_per_s'],
+                          unit='photons/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
+                          ms_per_step=out['ms_per_step'], higher_is_better=True, scaling='weak',
+                          vs_baseline=None, dtype='f32', data='synthetic',
+                          config={'workload': out['workload'], 'photons': out['photons'], 'form': 'persistent',
+                                  'tuning': out['tuning']}, detail=out)), flush=True)
 
 
 def main():
@@ -286,87 +361,68 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=10)
     ap.add_argument('--warmup', type=int, default=2)
-    ap.add_argument('--quick', action='store_true', help='skip serving, CPU baseline and instanced-CSG legs')
+    ap.add_argument('--quick', action='store_true', help='headline only: no CPU, serving, secondary or precision legs')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-serving', action='store_true')
     ap.add_argument('--no-precision', action='store_true', help='skip the thin-shell precision and history-chi2 legs')
-    ap.add_argument('--workload', choices=['csi_calorimeter', 'instanced_csg'], default='csi_calorimeter',
-                    help='instanced_csg: only the timed 10M-photon instanced-CSG launches (for profiling that '
-                         'kernel); the default is the full bench on the calorimeter event')
+    ap.add_argument('--workload', choices=sorted(WORKLOADS), default='instanced_csg')
+    ap.add_argument('--only', action='store_true', help='time only the --workload launches (for profiling)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one MI355X: this engine runs one process on one GPU and the '
               'multi-GPU form is a design note only; run with --gpus 1', file=sys.stderr)
         sys.exit(2)
     ensure_built()
-    from oxr import abi, scenes, gensteps as G
+    from oxr import abi, scenes
     from oxr import engine as engine_mod
-
-    if args.workload == 'instanced_csg':
-        instanced_only(engine_mod, abi, scenes, args)
+    if args.only:
+        only(engine_mod, abi, scenes, args)
         return
+    threads = host_threads()
 
-    # ---------------------------------------------------------------- headline
-    name = 'csi_calorimeter'
-    s = scenes.csi_calorimeter()
-    gs = s.gensteps()
-    cfg = abi.default_config(**s.config)
-    e = engine_mod.Engine(s.geometry, s.physics, cfg)
-    info = e.info()
-    tune = load_tuning(name)
-    if tune:
-        e.set_tuning(**tune)
-    n = e.set_gensteps(gs)
-    log(f'[bench] {info["device_name"]} {info["arch"]} CUs={info["num_cu"]} workload={name} photons={n} tuning={e.tuning()}')
-    dt, st, kms = timed_launches(e, abi.OXR_FORM_PERSISTENT, args.steps, args.warmup)
-    value = args.steps * n / dt
-    steps_per_s = args.steps * st['bounce_steps'] / dt
-    log(f'[bench] persistent: {value / 1e6:.1f} M photons/s, {steps_per_s / 1e6:.1f} M steps/s, kernel {kms:.2f} ms')
-    dtp, stp, kmsp = timed_launches(e, abi.OXR_FORM_PLAIN, args.steps, args.warmup)
-    plain_value = args.steps * n / dtp
-    log(f'[bench] plain:      {plain_value / 1e6:.1f} M photons/s, kernel {kmsp:.2f} ms')
-    cnt = e.launch(counting=True)
-    ab = algorithmic_bytes(cnt, n, len(gs), st['num_hit'])
-    achieved = ab / (kms / 1e3) / 1e9
-    prof = read_profile(name)
-    roof = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=prof.get('traffic'),
-                kernel='k_generate + k_simulate_persist (one simulate launch)',
-                generate_ms=round(st['generate_ms'], 4), propagate_ms=round(kms - st['generate_ms'], 4),
-                valu_busy_pct=prof.get('valu_busy'), valu_lane_util_pct=prof.get('valu_util'),
-                algorithmic_bytes_per_launch=int(ab),
-                node_visits=int(cnt['node_visits']), prim_tests=int(cnt['prim_tests']),
-                csg_node_bytes=int(cnt['csg_node_bytes']), instance_visits=int(cnt['instance_visits']))
-    log(f'[bench] roofline: {achieved:.0f} GB/s algorithmic ({ab / 1e9:.2f} GB/launch)')
+    # ---------------------------------------------------------------- headline: instanced CSG
+    name = args.workload
+    head, (s3, cfg3, gs3, t3, inp3) = measure(engine_mod, abi, scenes, name, args.steps, args.warmup)
     out = {
         'metric': 'photons/s & bounce-steps/s on 1 MI355X; p99 event latency vs per-event launch',
-        'value': round(value, 1), 'unit': 'photons/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
-        'ms_per_step': round(1e3 * dt / args.steps, 3), 'higher_is_better': True, 'scaling': 'weak',
+        'value': head['photons_per_s'], 'unit': 'photons/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': head['ms_per_step'], 'higher_is_better': True, 'scaling': 'weak',
         'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': 'csi_calorimeter: 8x8 CsI 20x20x200 mm crystals + 6x6 mm SiPMs, '
-                               '2000 scintillation + 500 Cerenkov gensteps, Poisson 1000/150 photons',
-                   'photons': n, 'gensteps': len(gs), 'form': 'persistent', 'tuning': e.tuning(),
-                   'max_bounce': cfg.max_bounce},
-        'bounce_steps_per_s': round(steps_per_s, 1),
-        'kernel_ms': round(kms, 3),
-        'plain_form': {'photons_per_s': round(plain_value, 1), 'kernel_ms': round(kmsp, 3)},
-        'roofline': roof,
+        'config': {'workload': head['workload'], 'photons': head['photons'], 'gensteps': head['gensteps'],
+                   'form': 'persistent', 'tuning': head['tuning'], 'max_bounce': head['max_bounce']},
+        'bounce_steps_per_s': head['bounce_steps_per_s'], 'kernel_ms': head['kernel_ms'],
+        'plain_form': head['plain_form'], 'persistent_vs_plain': head['persistent_vs_plain'],
+        'roofline': head['roofline'],
     }
-    e.set_gensteps(gs)
-    if not (args.quick or args.no_cpu):
-        cb = cpu_baseline(s, gs, cfg)
-        out['cpu_baseline'] = cb
-        log(f"[bench] cpu baseline: {cb['value']:.0f} photons/s on {cb['cores']} threads")
-    else:
-        out['cpu_baseline'] = None
+    out['cpu_baseline'] = None if (args.quick or args.no_cpu) else cpu_baseline(s3, gs3, cfg3, threads, inp=inp3)
+    if out['cpu_baseline']:
+        log(f"[bench] cpu baseline: {out['cpu_baseline']['value']:.0f} photons/s on {threads} threads")
+    if args.quick:
+        print(json.dumps(out), flush=True)
+        return
 
-    if not (args.quick or args.no_serving):
-        # ------------------------------------------------------------ serving (C2 geometry)
-        rng = np.random.default_rng(42)
-        evs, counts = serving_events(rng, 256, 200_000, 1.0, 1_000, 20_000_000, 60.0)
-        arrivals = np.cumsum(rng.exponential(1 / 40.0, size=256))
+    # ---------------------------------------------------------------- other workloads
+    work = {}
+    for w in ('csi_calorimeter', 'nested_boxes'):
+        if w == name:
+            continue
+        r, (s, cfg, gs, tune, inp) = measure(engine_mod, abi, scenes, w, args.steps, args.warmup)
+        if not args.no_cpu:
+            r['cpu_baseline'] = cpu_baseline(s, gs, cfg, threads, budget_s=5.0)
+        work[w] = r
+    out['workloads'] = work
+
+    if not args.no_serving:
+        # ------------------------------------------------------------ serving stream (C2 geometry)
+        s = scenes.csi_calorimeter()
+        e = engine_mod.Engine(s.geometry, s.physics, abi.default_config(**s.config))
+        tune = load_tuning('csi_calorimeter')
+        if tune:
+            e.set_tuning(**tune)
+        evs, counts, arrivals = scenes.serving_stream(np.random.default_rng(42))
         lat_s, wall_s, ph_s, sst = run_server(engine_mod, e, evs, arrivals, 4 << 20)
         lat_b, wall_b, ph_b, busy_b = run_sequential(e, evs, arrivals)
+        e.close()
         out['serving'] = {
             'workload': '256 events, Poisson arrivals 40/s, log-normal photons (median 200k, sigma 1, '
                         'clipped 1k-20M), torch gensteps, csi_calorimeter geometry resident',
@@ -382,56 +438,42 @@ def main():
                                 'photons_per_s_busy': round(ph_b / busy_b, 1)},
         }
         log(f"[bench] serving: {out['serving']}")
-    e.close()
-    threads = min(16, os.cpu_count() or 1)
-    do_prec = not (args.quick or args.no_precision)
-    chi2 = {}
-    if do_prec:
-        chi2['csi_calorimeter'] = history_chi2_leg(engine_mod, s, gs, cfg, 50_000, threads, tune=tune)
-        s1 = scenes.nested_boxes()
-        chi2['nested_boxes'] = history_chi2_leg(engine_mod, s1, s1.gensteps(), abi.default_config(**s1.config),
-                                                100_000, threads)
-        log(f"[bench] history chi2: {chi2}")
-
-    if not args.quick:
-        # ------------------------------------------------------------ instanced CSG
-        s3 = scenes.instanced_csg()
-        cfg3 = abi.default_config(**s3.config)
+        # ------------------------------------------------------------ served vs per-event, instanced CSG
         e3 = engine_mod.Engine(s3.geometry, s3.physics, cfg3)
-        t3 = load_tuning('instanced_csg')
         if t3:
             e3.set_tuning(**t3)
-        gs3 = s3.gensteps()
-        n3 = e3.set_gensteps(gs3)
-        dt3, st3, k3 = timed_launches(e3, abi.OXR_FORM_PERSISTENT, 3, 1)
-        dt3p, st3p, k3p = timed_launches(e3, abi.OXR_FORM_PLAIN, 3, 1)
-        rng = np.random.default_rng(43)
-        evs, counts = serving_events(rng, 128, 200_000, 1.0, 1_000, 20_000_000, 16000.0)
+        evs, counts, _ = scenes.serving_stream(np.random.default_rng(43), n_events=128, ext=16000.0,
+                                               wavelength=430.0)
         zero = np.zeros(len(evs))
         lat_s, wall_s, ph_s, sst = run_server(engine_mod, e3, evs, zero, 8 << 20)
         lat_b, wall_b, ph_b, busy_b = run_sequential(e3, evs, zero)
-        out['instanced_csg'] = {
-            'workload': 'water sphere r=19.5 m, acrylic shell, 20000 instanced 7-node CSG solids, 10M torch photons',
-            'photons': n3,
-            'persistent_photons_per_s': round(3 * n3 / dt3, 1), 'persistent_kernel_ms': round(k3, 2),
-            'plain_photons_per_s': round(3 * n3 / dt3p, 1), 'plain_kernel_ms': round(k3p, 2),
-            'bounce_steps': int(st3['bounce_steps']),
-            'served_vs_per_event': {
-                'events': len(evs), 'photons': int(counts.sum()),
-                'served_photons_per_s': round(ph_s / wall_s, 1),
-                'per_event_plain_photons_per_s': round(ph_b / wall_b, 1),
-                'ratio': round((ph_s / wall_s) / (ph_b / wall_b), 3),
-                'served_p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 1),
-                'per_event_p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 1)},
-        }
-        log(f"[bench] instanced_csg: {out['instanced_csg']}")
         e3.close()
-        if do_prec:
-            chi2['instanced_csg'] = history_chi2_leg(engine_mod, s3, gs3, cfg3, 20_000, threads, tune=t3)
-    if do_prec:
+        out['served_instanced_csg'] = {
+            'workload': '128 events submitted at once, log-normal photons (median 200k, sigma 1, clipped 1k-20M), '
+                        'torch gensteps at random interior origins, instanced_csg geometry resident',
+            'events': len(evs), 'photons': int(counts.sum()),
+            'served_photons_per_s': round(ph_s / wall_s, 1),
+            'per_event_plain_photons_per_s': round(ph_b / wall_b, 1),
+            'ratio': round((ph_s / wall_s) / (ph_b / wall_b), 3),
+            'served_p50_ms': round(1e3 * float(np.percentile(lat_s, 50)), 1),
+            'served_p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 1),
+            'per_event_p50_ms': round(1e3 * float(np.percentile(lat_b, 50)), 1),
+            'per_event_p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 1),
+            'served_steps': int(sst['steps'])}
+        log(f"[bench] served instanced_csg: {out['served_instanced_csg']}")
+
+    if not args.no_precision:
+        chi2 = {}
         prec, (s5, gs5, inp5, ng5) = precision_leg(engine_mod, threads)
         out['thin_shells_precision'] = prec
         log(f"[bench] thin shells precision: {prec}")
+        chi2['instanced_csg'] = history_chi2_leg(engine_mod, s3, gs3, cfg3, 20_000, threads, tune=t3)
+        s2 = scenes.csi_calorimeter()
+        chi2['csi_calorimeter'] = history_chi2_leg(engine_mod, s2, s2.gensteps(), abi.default_config(**s2.config),
+                                                   50_000, threads, tune=load_tuning('csi_calorimeter'))
+        s1 = scenes.nested_boxes()
+        chi2['nested_boxes'] = history_chi2_leg(engine_mod, s1, s1.gensteps(), abi.default_config(**s1.config),
+                                                100_000, threads)
         chi2['thin_shells_isotropic'] = history_chi2_leg(engine_mod, s5, gs5, abi.default_config(**s5.config),
                                                          20_000, threads, tune=load_tuning('thin_shells'),
                                                          inp=inp5, begin=ng5)

@@ -647,9 +647,14 @@ OXD bool intersect_leaf(Is& is, const float4* node, int ni, const float4* plan, 
         default: break;
     }
     if (valid && gt > 0) {
-        float x = q0.x * is.x + q0.y * is.y + q0.z * is.z;
-        float y = q1.x * is.x + q1.y * is.y + q1.z * is.z;
-        float z = q2.x * is.x + q2.y * is.y + q2.z * is.z;
+        // the transform rows are read again rather than held through the leaf: fewer
+        // registers live across the leaf arithmetic (the same values, from the cache)
+        asm volatile("" ::: "memory");
+        const float4* q = itra + 4 * (gt - 1);
+        const float4 r0 = q[0], r1 = q[1], r2 = q[2];
+        float x = r0.x * is.x + r0.y * is.y + r0.z * is.z;
+        float y = r1.x * is.x + r1.y * is.y + r1.z * is.z;
+        float z = r2.x * is.x + r2.y * is.y + r2.z * is.z;
         is.x = x; is.y = y; is.z = z;
     }
     if (complement) {
@@ -813,22 +818,79 @@ OXD bool intersect_node(Is& is, const float4* node, int ni, int root, const floa
 
 // intersect_tree (CSG/csg_intersect_tree.h:277-672): postorder evaluation with a tranche
 // stack of subtree slices and a stack of partial intersects; side recorded in sign(t).
-template <bool FP64, bool LIST>
+// The general form keeps the reference's capacities, 15 partial intersects and 4
+// tranches, in private arrays. Over every sequence of LUT decisions a tree of height h
+// holds at most h+1 partial intersects, and a tree of height <= 2 at most 3 tranches:
+// with SHALLOW (every tree of the geometry has height <= 2) both stacks are three named
+// local entries each, selected by if-chains and never indexed, so they live in registers
+// and no private memory is touched. The overflow checks keep the reference's meaning:
+// with these capacities they cannot fire for such trees, as they cannot with its own.
+template <bool FP64, bool LIST, bool SHALLOW = false>
 OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     const unsigned numNode = __float_as_uint(node[4 * root].x);
     const unsigned height = unsigned(__ffs(int(numNode + 1)) - 2);
     const float propagate_epsilon = 0.0001f;
-    float tr_tmin[4];
-    unsigned tr_slice[4];
+    constexpr int NS = SHALLOW ? 3 : 15, NT = SHALLOW ? 3 : 4;
+    Is st[SHALLOW ? 1 : 15];                       // general form
+    unsigned tr_slice[SHALLOW ? 1 : 4];
+    float tr_tmin[SHALLOW ? 1 : 4];
+    Is s0, s1, s2;                                 // SHALLOW form
+    s0.x = s0.y = s0.z = s0.w = 0.f; s1 = s0; s2 = s0;
+    unsigned c0 = 0u, c1 = 0u, c2 = 0u;
+    float m0 = 0.f, m1 = 0.f, m2 = 0.f;
+    // SHALLOW accessors: every entry is written on every update (a select per component),
+    // so no store depends on control flow and no entry ever needs an address
+    auto put = [&](int i, const Is& v) __attribute__((always_inline)) {
+        if constexpr (SHALLOW) {
+            const bool k0 = i == 0, k1 = i == 1, k2 = i >= 2;
+            s0.x = k0 ? v.x : s0.x; s0.y = k0 ? v.y : s0.y; s0.z = k0 ? v.z : s0.z; s0.w = k0 ? v.w : s0.w;
+            s1.x = k1 ? v.x : s1.x; s1.y = k1 ? v.y : s1.y; s1.z = k1 ? v.z : s1.z; s1.w = k1 ? v.w : s1.w;
+            s2.x = k2 ? v.x : s2.x; s2.y = k2 ? v.y : s2.y; s2.z = k2 ? v.z : s2.z; s2.w = k2 ? v.w : s2.w;
+        } else {
+            st[i] = v;
+        }
+    };
+    auto take = [&](int i) __attribute__((always_inline)) -> Is {
+        Is r;
+        if constexpr (SHALLOW) {
+            const float x01 = i == 0 ? s0.x : s1.x, y01 = i == 0 ? s0.y : s1.y;
+            const float z01 = i == 0 ? s0.z : s1.z, w01 = i == 0 ? s0.w : s1.w;
+            r.x = i >= 2 ? s2.x : x01; r.y = i >= 2 ? s2.y : y01; r.z = i >= 2 ? s2.z : z01; r.w = i >= 2 ? s2.w : w01;
+        } else {
+            r = st[i];
+        }
+        return r;
+    };
+    auto push_tr = [&](int i, unsigned c, float m) __attribute__((always_inline)) {
+        if constexpr (SHALLOW) {
+            const bool k0 = i == 0, k1 = i == 1, k2 = i >= 2;
+            c0 = k0 ? c : c0; m0 = k0 ? m : m0;
+            c1 = k1 ? c : c1; m1 = k1 ? m : m1;
+            c2 = k2 ? c : c2; m2 = k2 ? m : m2;
+        } else {
+            tr_slice[i] = c; tr_tmin[i] = m;
+        }
+    };
+    auto pick = [](bool c, const Is& a, const Is& b) __attribute__((always_inline)) -> Is {
+        Is r;
+        r.x = c ? a.x : b.x; r.y = c ? a.y : b.y; r.z = c ? a.z : b.z; r.w = c ? a.w : b.w;
+        return r;
+    };
     int tr = -1;
-    Is st[15];
     int sp = -1;
     int ierr = 0;
-    tr++; tr_slice[tr] = ((1u << height) & 0xffu) << 16; tr_tmin[tr] = t_min;
+    tr++; push_tr(tr, ((1u << height) & 0xffu) << 16, t_min);
     while (tr > -1) {
-        unsigned slice = tr_slice[tr];
-        float tmin = tr_tmin[tr];
+        unsigned slice;
+        float tmin;
+        if constexpr (SHALLOW) {
+            slice = tr >= 2 ? c2 : (tr == 1 ? c1 : c0);
+            tmin = tr >= 2 ? m2 : (tr == 1 ? m1 : m0);
+        } else {
+            slice = tr_slice[tr];
+            tmin = tr_tmin[tr];
+        }
         tr--;
         unsigned nodeIdx = (slice >> 16) & 0xff, endIdx = (slice >> 24) & 0xff;
         while (nodeIdx != endIdx) {
@@ -844,16 +906,15 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
                 Is ni_is; ni_is.x = 0.f; ni_is.y = 0.f; ni_is.z = 0.f; ni_is.w = 0.f;
                 intersect_node<FP64, LIST>(ni_is, node, ni, root, plan, itra, tmin, ro, rd);
                 ni_is.w = copysignf(ni_is.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
-                if (sp >= 14) { ierr = 1; break; }
-                st[++sp] = ni_is;
+                if (sp >= NS - 1) { ierr = 1; break; }
+                put(++sp, ni_is);
             } else {
                 if (sp < 1) { ierr = 1; break; }
-                bool firstLeft = signbit(st[sp].w);
-                bool secondLeft = signbit(st[sp - 1].w);
+                const Is top0 = take(sp), top1 = take(sp - 1);
+                bool firstLeft = signbit(top0.w);
+                bool secondLeft = signbit(top1.w);
                 if (!(firstLeft ^ secondLeft)) { ierr = 1; break; }
-                int left = firstLeft ? sp : sp - 1;
-                int right = firstLeft ? sp - 1 : sp;
-                Is L = st[left], R = st[right];
+                const Is L = pick(firstLeft, top0, top1), R = pick(firstLeft, top1, top0);
                 int l_state = classify(L, rd, tmin);
                 int r_state = classify(R, rd, tmin);
                 float t_left = fabsf(L.w), t_right = fabsf(R.w);
@@ -864,29 +925,27 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
                 if (l_promote) { l_state = 1; leftIsCloser = false; }
                 int ctrl = csg_lut(tc, l_state, r_state, leftIsCloser);
                 if (ctrl < 4) {
-                    Is result;
+                    Is result = pick(ctrl == 1, L, R);
                     if (ctrl == 0) { result.x = 0.f; result.y = 0.f; result.z = 0.f; result.w = 0.f; }
-                    else result = ctrl == 1 ? L : R;
                     if (ctrl == 3) { result.x = -result.x; result.y = -result.y; result.z = -result.z; }
                     result.w = copysignf(result.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
                     sp -= 2;
-                    st[++sp] = result;
+                    put(++sp, result);
                 } else {
                     bool loopA = ctrl == 4;
-                    Is loopside = loopA ? L : R;
-                    Is other = loopA ? R : L;
+                    const Is other = pick(loopA, R, L);
                     unsigned leftIdx = 2 * nodeIdx, rightIdx = leftIdx + 1;
-                    float tminAdvanced = fabsf(loopside.w) + propagate_epsilon;
+                    float tminAdvanced = fabsf(loopA ? L.w : R.w) + propagate_epsilon;
                     sp -= 2;
-                    st[++sp] = other;
+                    put(++sp, other);
                     unsigned endTree = ((nodeIdx & 0xffu) << 16) | ((endIdx & 0xffu) << 24);
                     unsigned leftTree = (((leftIdx << (elevation - 1)) & 0xffu) << 16) | (((rightIdx << (elevation - 1)) & 0xffu) << 24);
                     unsigned rightTree = (((rightIdx << (elevation - 1)) & 0xffu) << 16) | ((nodeIdx & 0xffu) << 24);
                     unsigned loopTree = loopA ? leftTree : rightTree;
-                    if (tr >= 3) { ierr = 1; break; }
-                    tr++; tr_slice[tr] = endTree; tr_tmin[tr] = tmin;
-                    if (tr >= 3) { ierr = 1; break; }
-                    tr++; tr_slice[tr] = loopTree; tr_tmin[tr] = tminAdvanced;
+                    if (tr >= NT - 1) { ierr = 1; break; }
+                    tr++; push_tr(tr, endTree, tmin);
+                    if (tr >= NT - 1) { ierr = 1; break; }
+                    tr++; push_tr(tr, loopTree, tminAdvanced);
                     break;
                 }
             }
@@ -895,18 +954,18 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
         if (ierr) break;
     }
     out.x = 0.f; out.y = 0.f; out.z = 0.f; out.w = 0.f;
-    if (sp == 0) out = st[0];
+    if (sp == 0) out = take(0);
     return out.w > 0.f;
 }
 
 // intersect_prim (CSG/csg_intersect_tree.h:684-719)
-template <bool FP64, bool LIST>
+template <bool FP64, bool LIST, bool SHALLOW = false>
 OXD bool intersect_prim(Is& is, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;   // zeroed isect of CSGOptiX7.cu:894-898
     unsigned tc = __float_as_uint(node[4 * root + 3].z);
     if (tc >= CSG_LEAF) return intersect_leaf<FP64>(is, node, root, plan, itra, t_min, ro, rd);
-    if (tc < CSG_NODE) return intersect_tree<FP64, LIST>(is, node, root, plan, itra, t_min, ro, rd);
+    if (tc < CSG_NODE) return intersect_tree<FP64, LIST, SHALLOW>(is, node, root, plan, itra, t_min, ro, rd);
     if (LIST) return intersect_node<FP64, LIST>(is, node, root, root, plan, itra, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     return false;

@@ -37,6 +37,7 @@ oxd::Geom Engine::geom() const {
     g.tlas = f ? tlas_f.p : tlas.p; g.blas = blas.p; g.inst = inst.p;
     g.flat = f ? flat.p : nullptr;
     g.tlas_root = f ? tlas_root_f : tlas_root; g.num_inst = num_inst;
+    g.vizmask = cfg.vizmask;
     return g;
 }
 
@@ -105,6 +106,9 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.ntop = ntop;
     P.stack_cap = stack_cap();
     P.tn_stack = tn ? 1 : 0;
+    P.shallow_csg = !has_list && csg_height <= 2 ? 1 : 0;
+    P.refine = cfg.propagate_refine ? 1 : 0;
+    P.refine_distance = cfg.refine_distance;
     P.refill_threshold = std::max(1, tune.refill_batch);
     P.segs = nullptr;
     P.nseg = 0;
@@ -263,12 +267,14 @@ static bool is_op(uint32_t tc) { return tc == 1 || tc == 2 || tc == 3; }
 // arrays: node ranges, tree shape (root subNum = 2^k-1 tree nodes, operators above the
 // leaf level), list nodes (CSG_CONTIGUOUS/DISCONTIGUOUS/OVERLAP: subs inside the prim and
 // all leaves, at most 8 for contiguous), leaf typecodes, transform and plane ranges,
-// boundary indices. Sets e.has_list when any prim holds a list node.
+// boundary indices. Sets e.has_list when any prim holds a list node and e.csg_height to
+// the greatest tree height.
 static int validate_and_prims(Engine& e, const oxr_geometry* g, const oxr_physics* ph, std::vector<int2>& primtab) {
     const uint32_t* nu = reinterpret_cast<const uint32_t*>(g->node);
     auto node_u = [&](int64_t i, int k) { return nu[16 * i + k]; };
     primtab.resize(size_t(g->num_prim));
     e.has_list = false;
+    e.csg_height = 0;
     for (int64_t p = 0; p < g->num_prim; p++) {
         const int32_t* pi = reinterpret_cast<const int32_t*>(g->prim + 16 * p);
         const uint32_t* pu = reinterpret_cast<const uint32_t*>(g->prim + 16 * p);
@@ -311,6 +317,7 @@ static int validate_and_prims(Engine& e, const oxr_geometry* g, const oxr_physic
                 return fail(OXR_EINVAL, tag + "tree root subNum must be 2^k-1 <= 255 and within numNode");
             int height = 0;
             while ((1 << (height + 1)) - 1 < int(T)) height++;
+            e.csg_height = std::max(e.csg_height, height);
             for (int i = 1; i <= int(T) && rc == OXR_OK; i++) {
                 int64_t ni = nodeOffset + i - 1;
                 uint32_t tc = node_u(ni, 14);
@@ -356,6 +363,7 @@ static bool config_ok(const oxr_config& c) {
     if (c.max_record < 0 || c.max_photon < 1 || c.max_rec < 0 || c.max_prd < 0) return false;
     // the rec domains divide each compressed coordinate
     if (c.max_rec > 0 && !(c.extent_domain > 0.f && c.time_domain > 0.f)) return false;
+    if (c.propagate_refine > 1u || !std::isfinite(c.refine_distance)) return false;
     return true;
 }
 
@@ -374,6 +382,8 @@ void oxr_config_defaults(oxr_config* c) {
     c->photonlite = 0;
     c->max_rec = 0; c->max_prd = 0;
     c->extent_domain = 1000.f; c->time_domain = 10.f;   // SEventConfig.cc:72-73
+    c->propagate_refine = 0u; c->refine_distance = 5000.f;   // SEventConfig.cc:105-106
+    c->vizmask = 0xffu;                                    // Params.cc:156
     c->reserved = 0;
 }
 
@@ -503,7 +513,11 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
         d.c1 = make_float4(float(W[0][1]), float(W[1][1]), float(W[2][1]), float(W[3][1]));
         d.c2 = make_float4(float(W[0][2]), float(W[1][2]), float(W[2][2]), float(W[3][2]));
         uint32_t ii_id = ((uint32_t(i) & 0xffffu) << 16) | (uint32_t(sid1) & 0xffffu);
-        d.info = make_int4(e->solid_prim_offset[size_t(gas)], int(ii_id), ident ? 1 : 0, e->solid_blas_root[size_t(gas)]);
+        // flags: bit 0 identity transform; bits 8-15 the instance visibility mask, one bit
+        // per solid up to the eighth (Properties::visibilityMask, Properties.cc:45-55)
+        const uint32_t vis = 1u << std::min<int32_t>(gas, 7);
+        d.info = make_int4(e->solid_prim_offset[size_t(gas)], int(ii_id), int((ident ? 1u : 0u) | (vis << 8)),
+                           e->solid_blas_root[size_t(gas)]);
         insts[size_t(i)] = d;
         // world box of the instance: solid box corners through the object->world transform
         const oxh::Box& sb = solid_box[size_t(gas)];
@@ -669,6 +683,8 @@ int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
     if (hipMemGetInfo(&fr, &tot) == hipSuccess) { info->free_bytes = int64_t(fr); info->total_bytes = int64_t(tot); }
     std::strncpy(info->device_name, e->name.c_str(), sizeof(info->device_name) - 1);
     std::strncpy(info->arch, e->arch.c_str(), sizeof(info->arch) - 1);
+    info->csg_height = e->csg_height;
+    info->csg_in_registers = !e->has_list && e->csg_height <= 2 ? 1 : 0;
     return OXR_OK;
 }
 

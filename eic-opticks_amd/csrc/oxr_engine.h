@@ -101,6 +101,7 @@ struct Engine {
     int64_t pre_rows = 0;
     int num_bnd = 0;
     bool has_list = false;   // some prim holds a list node: launches use the LIST kernels
+    int csg_height = 0;      // greatest CSG tree height over the prims (0: leaves and lists only)
     bool launched = false;
     bool hitlite_ready = false;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, eg = nullptr;

@@ -58,6 +58,9 @@ struct Params {
     int ntop;                  // top-level BVH nodes staged in LDS
     int stack_cap;             // per-lane traversal stack entries in LDS
     int tn_stack;              // stack entries carry their box entry distance (8 B each)
+    int shallow_csg;           // every CSG tree has height <= 2 and no prim holds a list node
+    int refine;                // PropagateRefine: trace again from 0.99 of a far hit
+    float refine_distance;     // PropagateRefineDistance (mm)
     int refill_threshold;      // persistent form: idle lanes per wave before a refill
     const Seg* segs;           // batched serving step, or null for a single event
     int nseg;

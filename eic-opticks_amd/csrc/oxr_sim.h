@@ -1,7 +1,8 @@
 // oxr_sim.h -- the per-photon pieces and the two simulate kernels (plain and persistent
 // form), templated on root precision (FP64), counting (COUNT), debug outputs (DBG), list-node
 // support (LIST) and the traversal mode TM (bit 0: stack entries carry box entry distances;
-// bit 1: the flattened BVH). k_generate (oxr_kernels.hip) uses the per-photon pieces; the
+// bit 1: the flattened BVH; bit 2: every CSG tree has height <= 2, so the CSG evaluation
+// stacks live in registers; production kernels without list nodes only). k_generate (oxr_kernels.hip) uses the per-photon pieces; the
 // kernels are instantiated by the four oxr_sim_*.hip units, one per (FP64, LIST) pair, which
 // compile in parallel.
 #pragma once
@@ -86,17 +87,47 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
     rng_init(L.r, P.rng_seed, P.rng_offset, P.skip, (unsigned long long)evt, pidx);
     if (DBG && P.pre) { L.r.pre = P.pre + (size_t)idx * P.npre; L.r.npre = P.npre; }
     uint32_t gencode = __float_as_uint(gs[0].x);
+    // the photon starts zeroed (sctx ctx = {}, CSGOptiX7.cu:432): a generator that leaves a
+    // field unset (T_RECTANGLE beyond its fourth side sets no position or direction,
+    // sysrap/storch.h:460-512) leaves it zero
+    L.p = Ph{};
     if (gencode == 6) gen_torch(L.p, gs, pidx, L.r);
     else if (gencode == 15 || gencode == 18) gen_cerenkov(L.p, gs, L.r, P.t);
     else if (gencode == 16 || gencode == 5) gen_scint(L.p, gs, L.r, P.t);
     else if (gencode == 19) { load_photon(L.p, P.input + 4 * pidx); set_flag(L.p, TORCH); }
-    else { L.p = Ph{}; zero_flags(L.p); }
+    else { zero_flags(L.p); }
     set_index(L.p, pidx);
     L.sh0 = L.sh1 = L.sb0 = L.sb1 = 0ull;
     if (DBG) { L.lc = 0.f; L.lf = 0.f; }
     L.bounce = 0;
     point<DBG>(P, L);
     return true;
+}
+
+// trace<refine> (CSGOptiX7.cu:110-216): with PropagateRefine, when 0.99 of the distance
+// found lies beyond the refine distance, the ray is traced again from that point and the
+// 0.99 part added back, so a far surface is intersected from close by. A miss reports
+// distance 1, as the miss program writes it (CSGOptiX7.cu:666-670), so h.t is the
+// reference's prd distance in every case. One trace call site serves both passes.
+template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW>
+OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int* stk, int stride,
+                       const BNode* top, int ntop, uint32_t& visits, uint32_t& tests, uint32_t& err,
+                       uint32_t& csgb, uint32_t& ivis) {
+    f3 o = pos;
+    float t_approx = 0.f;
+    bool refined = false, found;
+    for (;;) {
+        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW>(P.g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap, top,
+                                                           ntop, visits, tests, err, csgb, ivis);
+        if (!found) h.t = 1.f;
+        if (!P.refine || refined) break;
+        t_approx = 0.99f * h.t;
+        if (!(t_approx > P.refine_distance)) break;
+        o = mk(pos.x + t_approx * mom.x, pos.y + t_approx * mom.y, pos.z + t_approx * mom.z);
+        refined = true;
+    }
+    if (refined) h.t = h.t + t_approx;
+    return found;
 }
 
 // one loop iteration of CSGOptiX7.cu:450-482; returns false when the photon is done
@@ -106,8 +137,8 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return false;
     float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Hit h;
-    bool found = trace<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0>(P.g, L.p.pos, L.p.mom, tmin, P.tmax, h, stk, stride, P.stack_cap, top, ntop,
-                                    visits, tests, err, csgb, ivis);
+    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0>(
+        P, L.p.pos, L.p.mom, tmin, h, stk, stride, top, ntop, visits, tests, err, csgb, ivis);
     if (!found) {
         if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0
         return false;
@@ -278,7 +309,7 @@ template <bool FP64, bool LIST>
 hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, int kind, int tm) {
 #define OXR_PLAIN(C, D, T) hipLaunchKernelGGL((oxd::k_simulate<FP64, C, D, LIST, T>), g, b, sm, s, P)
     if (kind == SIM_COUNT) {
-        switch (tm) {
+        switch (tm & 3) {
             case 0: OXR_PLAIN(true, true, 0); break;
             case 1: OXR_PLAIN(true, true, 1); break;
             case 2: OXR_PLAIN(true, true, 2); break;
@@ -292,7 +323,18 @@ hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, di
             case 0: OXR_PLAIN(false, false, 0); break;
             case 1: OXR_PLAIN(false, false, 1); break;
             case 2: OXR_PLAIN(false, false, 2); break;
-            default: OXR_PLAIN(false, false, 3); break;
+            case 3: OXR_PLAIN(false, false, 3); break;
+            default:
+                if constexpr (!LIST) {
+                    switch (tm) {
+                        case 4: OXR_PLAIN(false, false, 4); break;
+                        case 5: OXR_PLAIN(false, false, 5); break;
+                        case 6: OXR_PLAIN(false, false, 6); break;
+                        default: OXR_PLAIN(false, false, 7); break;
+                    }
+                } else {
+                    return hipErrorInvalidValue;
+                }
         }
     }
 #undef OXR_PLAIN
@@ -310,7 +352,18 @@ hipError_t launch_simulate_persist(hipStream_t s, const oxd::Params& P, dim3 g, 
             case 0: OXR_PERSIST(false, 0); break;
             case 1: OXR_PERSIST(false, 1); break;
             case 2: OXR_PERSIST(false, 2); break;
-            default: OXR_PERSIST(false, 3); break;
+            case 3: OXR_PERSIST(false, 3); break;
+            default:
+                if constexpr (!LIST) {
+                    switch (tm) {
+                        case 4: OXR_PERSIST(false, 4); break;
+                        case 5: OXR_PERSIST(false, 5); break;
+                        case 6: OXR_PERSIST(false, 6); break;
+                        default: OXR_PERSIST(false, 7); break;
+                    }
+                } else {
+                    return hipErrorInvalidValue;
+                }
         }
     }
 #undef OXR_PERSIST

@@ -19,7 +19,11 @@
 namespace oxd {
 
 struct BNode { float4 a, b, c; int4 ch; };
-struct InstD { float4 c0, c1, c2; int4 info; };   // info: solid, ii_id, flags(bit0 identity), blas_root
+struct InstD { float4 c0, c1, c2; int4 info; };   // info: prim offset, ii_id, flags (bit 0 identity,
+                                                  // bits 8-15 visibility mask), blas_root
+// an instance takes part in a trace when its visibility mask shares a bit with the ray's
+// (OptixInstance::visibilityMask against optixTrace's visibilityMask, SBT.cc:538-545)
+OXD bool visible(const InstD& in, unsigned vizmask) { return ((unsigned(in.info.z) >> 8) & vizmask) != 0u; }
 struct SolidD { int primOffset, numPrim, blasRoot, pad; };
 
 struct Geom {
@@ -33,6 +37,7 @@ struct Geom {
     const int2* flat;      // flattened leaves (instance, local prim), or null for two levels
     int tlas_root;
     int num_inst;
+    unsigned vizmask;      // ray visibility mask against InstD flag bits 8-15
 };
 
 constexpr int SENTINEL = int(0x80000000u);
@@ -77,6 +82,7 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 }
 
 // Closest hit in (tmin, tmax]; ties in t go to the smaller (instance, local prim) key.
+// SHALLOW: every CSG tree of the geometry has height <= 2 (register-resident CSG stacks).
 // stk points at this lane's LDS stack column (element k at stk[k*stride]). With TN,
 // a second column of cap floats follows the stack (at stk + stride*cap) holding each
 // pushed node's box entry distance, and a popped node whose entry lies beyond the best
@@ -85,7 +91,7 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 // During traversal only (t, instance, local prim, object-frame normal) of the best hit
 // stay live; its world normal, local position and identity words are formed afterwards
 // from the instance record, with the same arithmetic as at instance entry.
-template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false>
+template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false, bool SHALLOW = false>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
@@ -151,6 +157,10 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             const int2 fi = g.flat[item];
             const InstD in = g.inst[fi.x];
             if (COUNT) ivis++;
+            if (!visible(in, g.vizmask)) {
+                if (!pop()) break;
+                continue;
+            }
             f3 lo = pos, ld = mom;
             if ((in.info.z & 1) == 0) to_object(in, pos, mom, lo, ld);
             int nodeOffset = g.prim[in.info.x + fi.y].x;
@@ -165,7 +175,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                 }
             }
             Is is;
-            if (intersect_prim<FP64, LIST>(is, g.node, nodeOffset, g.plan, g.itra, tmin, lo, ld)) {
+            if (intersect_prim<FP64, LIST, SHALLOW>(is, g.node, nodeOffset, g.plan, g.itra, tmin, lo, ld)) {
                 float t = is.w;
                 bool closer = t >= tmin && (t < best || (found && t == best && (fi.x < best_inst || (fi.x == best_inst && fi.y < best_item))));
                 if (closer) {
@@ -180,6 +190,10 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         if (at_top) {
             const InstD in = g.inst[item];
             if (COUNT) ivis++;
+            if (!visible(in, g.vizmask)) {
+                if (!pop()) break;
+                continue;
+            }
             inst = item;
             primOffset = in.info.x;
             if ((in.info.z & 1) == 0) {
@@ -211,7 +225,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             }
         }
         Is is;
-        if (intersect_prim<FP64, LIST>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
+        if (intersect_prim<FP64, LIST, SHALLOW>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
             // a reported distance outside the ray interval is ignored, as by
             // optixReportIntersection (CSGOptiX/CSGOptiX7.cu:912): a convex polyhedron
             // wholly behind the ray reports valid with the zeroed isect

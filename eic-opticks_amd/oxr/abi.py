@@ -51,6 +51,7 @@ class Config(C.Structure):
                 ("fp64_roots", C.c_int32), ("seq_enabled", C.c_int32), ("max_record", C.c_int32),
                 ("max_photon", C.c_int64), ("photonlite", C.c_int32), ("max_rec", C.c_int32),
                 ("max_prd", C.c_int32), ("extent_domain", C.c_float), ("time_domain", C.c_float),
+                ("propagate_refine", C.c_uint32), ("refine_distance", C.c_float), ("vizmask", C.c_uint32),
                 ("reserved", C.c_int32)]
 
 
@@ -72,7 +73,8 @@ class EngineInfo(C.Structure):
                 ("num_inst", C.c_int64), ("tlas_nodes", C.c_int64), ("blas_nodes", C.c_int64),
                 ("node_bytes", C.c_int32), ("num_cu", C.c_int32), ("device_bytes", C.c_int64),
                 ("free_bytes", C.c_int64), ("total_bytes", C.c_int64),
-                ("device_name", C.c_char * 64), ("arch", C.c_char * 32)]
+                ("device_name", C.c_char * 64), ("arch", C.c_char * 32),
+                ("csg_height", C.c_int32), ("csg_in_registers", C.c_int32)]
 
 
 class ServerConfig(C.Structure):
@@ -150,6 +152,9 @@ def default_config(**kw):
     c.max_prd = 0
     c.extent_domain = 1000.0    # SEventConfig MaxExtentDomain (mm)
     c.time_domain = 10.0        # SEventConfig MaxTimeDomain (ns)
+    c.propagate_refine = 0      # SEventConfig PropagateRefine
+    c.refine_distance = 5000.0  # SEventConfig PropagateRefineDistance (mm)
+    c.vizmask = 0xff            # Params vizmask (all eight instance visibility bits)
     c.reserved = 0
     for k, v in kw.items():
         setattr(c, k, v)

@@ -357,5 +357,60 @@ def thin_shell_grazing_photons(scene, n, seed=42):
     return p
 
 
+def serving_stream(rng, n_events=256, median=200_000, sigma=1.0, lo=1_000, hi=20_000_000, ext=60.0,
+                   rate=40.0, wavelength=420.0):
+    """The serving workload's request stream: n_events events arriving as a Poisson
+    process at `rate` per second, photons per event log-normal (median, sigma of the log)
+    clipped to [lo, hi], each event a batch of isotropic torch gensteps (one per 20k
+    photons, at most 64) at random origins in the cube |x|,|y|,|z| < ext mm.
+    Returns (gensteps per event, photon counts, arrival times in s)."""
+    counts = np.clip(np.exp(np.log(median) + sigma * rng.normal(size=n_events)), lo, hi).astype(np.int64)
+    evs = []
+    for c in counts:
+        ngs = int(max(1, min(64, c // 20000)))
+        pos = rng.uniform(-ext, ext, size=(ngs, 3))
+        per = np.full(ngs, c // ngs)
+        per[0] += c - per.sum()
+        evs.append(G.torch(per, pos=pos, wavelength=wavelength, type=G.T_POINT_ISOTROPIC))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_events))
+    return evs, counts, arrivals
+
+
 SCENES = dict(nested_boxes=nested_boxes, csi_calorimeter=csi_calorimeter,
               instanced_csg=instanced_csg, thin_shells=thin_shells, raindrop=raindrop)
+
+
+def hidden_instances(num_hidden=40, hide=True):
+    """A water ball in a black box with small glass spheres (solid 1) and glass cubes
+    (solid 2) placed around it; the cubes are the last instances. With hide=False the
+    cubes are left out of the geometry altogether, so that a launch whose visibility mask
+    drops solid 2's bit can be compared with a geometry that never held them."""
+    f = Foundry()
+    f.add_material('Rock', 1.0)
+    f.add_material('Water', 1.333, absorption_length=20_000.0, scattering_length=5_000.0)
+    f.add_material('Glass', 1.5, absorption_length=1_000.0)
+    f.add_implicit_absorber('Implicit_Water_Rock')
+    b_world = f.boundary('Rock', None, 'Implicit_Water_Rock', 'Water')
+    b_glass = f.boundary('Water', None, None, 'Glass')
+    f.begin_solid('r0')
+    f.add_prim(box3(2000.0), b_world)
+    f.end_solid()
+    f.begin_solid('r1')
+    f.add_prim(sphere(40.0), b_glass)
+    f.end_solid()
+    f.begin_solid('r2')
+    f.add_prim(box3(60.0), b_glass)
+    f.end_solid()
+    f.add_instance(np.eye(4), 0)
+    rng = np.random.default_rng(11)
+    for _ in range(40):
+        f.add_instance(translate(*rng.uniform(-700, 700, size=3)), 1)
+    if hide:
+        for _ in range(num_hidden):
+            f.add_instance(translate(*rng.uniform(-700, 700, size=3)), 2)
+
+    def gs(rng, s):
+        n = max(1, int(round(50_000 * s)))
+        return G.torch(n, pos=(0.0, 0.0, 0.0), wavelength=420.0, type=G.T_POINT_ISOTROPIC)
+
+    return _finish('hidden_instances' if hide else 'without_cubes', f, dict(max_bounce=32), gs)

@@ -144,6 +144,14 @@ typedef struct oxr_config {
     float    extent_domain;   /* rec position domain, mm, centred on the origin
                                  (SEventConfig MaxExtentDomain, default 1000)         */
     float    time_domain;     /* rec time domain, ns (MaxTimeDomain, default 10)      */
+    uint32_t propagate_refine;/* 1: a trace whose distance d has 0.99 d beyond
+                                 refine_distance is traced again from 0.99 d along the
+                                 ray and 0.99 d added back (SEventConfig PropagateRefine,
+                                 default 0; trace<true>, CSGOptiX7.cu:165-187)        */
+    float    refine_distance; /* mm, PropagateRefineDistance, default 5000            */
+    uint32_t vizmask;         /* ray visibility mask: an instance of solid s is seen
+                                 when vizmask & (1 << min(s, 7)) (Params.cc:156,
+                                 Properties.cc:45-55, SBT.cc:538-545); default 0xff  */
     int32_t  reserved;        /* 0                                                    */
 } oxr_config;
 
@@ -188,6 +196,9 @@ typedef struct oxr_engine_info {
     int64_t free_bytes, total_bytes;
     char    device_name[64];
     char    arch[32];
+    int32_t csg_height;       /* greatest CSG tree height over the prims (0: leaves, lists)  */
+    int32_t csg_in_registers; /* 1: production launches keep the CSG evaluation stacks in
+                                 registers (every tree of height <= 2, no list nodes)        */
 } oxr_engine_info;
 
 typedef struct oxr_engine oxr_engine;

@@ -974,6 +974,9 @@ bool trace(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
         int32_t sid1 = 0; std::memcpy(&sid1, q + 11, 4);
         const int32_t* so = c.solid + 12 * size_t(gas);
         int numPrim = so[4], primOffset = so[5];
+        // instance visibility mask, one bit per solid up to the eighth (Properties.cc:45-55,
+        // SBT.cc:538-545), against the ray's mask: OptiX skips an instance sharing no bit
+        if (((1u << std::min<int32_t>(gas, 7)) & c.cfg.vizmask) == 0u) continue;
         bool ident = c.inst_identity_transform[i] != 0;
         if (c.cull && !box_may_hit(&c.inst_box[6 * size_t(i)], double(pos.x), double(pos.y), double(pos.z),
                                    double(mom.x), double(mom.y), double(mom.z), double(best))) continue;
@@ -1032,6 +1035,24 @@ bool trace(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
     prd.iindex_identity = best_ii;
     prd.gp_boundary = best_gpbd;
     return true;
+}
+
+// trace<refine> (CSGOptiX/CSGOptiX7.cu:110-216) with the miss program's prd distance 1
+// (CSGOptiX7.cu:655-682, WITH_PRD): with PropagateRefine, when t_approx = 0.99 * distance
+// exceeds PropagateRefineDistance the ray is traced again from pos + t_approx * mom and
+// t_approx is added to the second distance (quad2::distance_add, sysrap/squad.h:235).
+template <typename T>
+bool trace_refined(const Ctx& c, V3<T> pos, V3<T> mom, T tmin, T tmax, PRD<T>& prd) {
+    bool found = trace<T>(c, pos, mom, tmin, tmax, prd);
+    if (!found) prd.t = T(1);
+    if (!c.cfg.propagate_refine) return found;
+    const T t_approx = T(0.99) * prd.t;
+    if (!(t_approx > T(c.cfg.refine_distance))) return found;
+    const V3<T> closer = v3<T>(pos.x + t_approx * mom.x, pos.y + t_approx * mom.y, pos.z + t_approx * mom.z);
+    found = trace<T>(c, closer, mom, tmin, tmax, prd);
+    if (!found) prd.t = T(1);
+    prd.t = prd.t + t_approx;
+    return found;
 }
 
 // ---------------------------------------------------------------- photon
@@ -1601,7 +1622,7 @@ void simulate_one(const Ctx& c, const float* gs_all, const int32_t* seed, int64_
     while (bounce < c.cfg.max_bounce && p.time < T(c.cfg.max_time)) {
         T tmin = (p.orient_boundary_flag & c.cfg.eps0_mask) ? T(c.cfg.tmin0) : T(c.cfg.tmin);
         PRD<T> prd;
-        trace<T>(c, p.pos, p.mom, tmin, T(c.cfg.tmax), prd);
+        trace_refined<T>(c, p.pos, p.mom, tmin, T(c.cfg.tmax), prd);
         last_lposcost = prd.lposcost; last_lposfphi = prd.lposfphi;
         if ((prd.gp_boundary & 0xffffu) == 0xffffu) break;
         V3<T> normal = normalize(prd.normal);
@@ -1765,7 +1786,8 @@ void orc_xlog(const float* x, int64_t n, float* out) {
 }
 
 // simtrace over FRAME gensteps: qsim::generate_photon_simtrace_frame (qudarap/qsim.h:2459-2510)
-// then trace and sevent::add_simtrace (sysrap/sevent.h:670-696); out (n,4,4). Returns n.
+// then trace<refine> (CSGOptiX7.cu:536-577) and sevent::add_simtrace (sysrap/sevent.h:670-696);
+// out (n,4,4). Returns n.
 int64_t orc_simtrace(void* cv, const float* gs, int64_t num_gs, float* out, int64_t cap) {
     const orc::Ctx& c = *static_cast<orc::Ctx*>(cv);
     int64_t total = orc_fill_seed(gs, num_gs, nullptr, 0);
@@ -1798,7 +1820,11 @@ int64_t orc_simtrace(void* cv, const float* gs, int64_t num_gs, float* out, int6
             v[k] = m[k] * d[0] + m[4 + k] * d[1] + m[8 + k] * d[2] + m[12 + k] * 0.f;
         }
         orc::PRD<float> r;
-        orc::trace<float>(c, orc::v3<float>(o[0], o[1], o[2]), orc::v3<float>(v[0], v[1], v[2]), c.cfg.tmin, c.cfg.tmax, r);
+        const bool found = orc::trace_refined<float>(c, orc::v3<float>(o[0], o[1], o[2]), orc::v3<float>(v[0], v[1], v[2]),
+                                                     c.cfg.tmin, c.cfg.tmax, r);
+        // a miss carries the miss program's q0: mid-grey background normal, distance 1
+        // (CSGOptiX7.cu:655-682, SBT::updateMiss SBT.cc:180-192)
+        if (!found) r.normal = orc::v3<float>(0.6f, 0.6f, 0.6f);
         const float t = r.t;
         float* a = out + 16 * size_t(i);
         a[0] = r.normal.x; a[1] = r.normal.y; a[2] = r.normal.z; a[3] = t;

@@ -26,10 +26,11 @@ def test_every_declared_symbol_is_exported(engine_mod):
 def test_struct_sizes_match_header(engine_mod):
     # sizes implied by include/oxr.h on LP64
     assert C.sizeof(abi.Geometry) == 96
-    assert C.sizeof(abi.Config) == 104
+    assert C.sizeof(abi.Config) == 120
     assert C.sizeof(abi.LaunchStats) == 88
     assert C.sizeof(abi.Tuning) == 24
     assert C.sizeof(abi.RequestResult) == 64
+    assert C.sizeof(abi.EngineInfo) == 184
 
 
 def test_config_defaults_are_reference_defaults(engine_mod):
@@ -46,6 +47,9 @@ def test_config_defaults_are_reference_defaults(engine_mod):
     assert c.max_time > 1e26
     # rec domains (SEventConfig.cc:72-73): 1000 mm, 10 ns; rec, prd and records off
     assert c.extent_domain == 1000.0 and c.time_domain == 10.0 and c.max_rec == 0 and c.max_prd == 0
+    # PropagateRefine off, PropagateRefineDistance 5000 mm (SEventConfig.cc:105-106); the
+    # ray visibility mask sees every instance (Params.cc:156)
+    assert c.propagate_refine == 0 and c.refine_distance == 5000.0 and c.vizmask == 0xff
     py = abi.default_config()
     for k, _ in abi.Config._fields_:
         assert getattr(py, k) == getattr(c, k), k

@@ -417,3 +417,30 @@ def test_debug_arrays_enabled_after_gensteps(engine_mod, oracle_lib):
     assert np.array_equal(bits(ph), bits(r['photon']))
     assert np.array_equal(seq, r['seq'])
     assert np.array_equal(bits(rec), bits(r['record']))
+
+
+def test_csg_stacks_in_registers_for_shallow_trees(engine_mod):
+    """Geometries whose CSG trees are at most two levels deep (the instanced-CSG solid,
+    height 2; the thin-shell solid, height 1; leaf-only scenes) run production kernels that
+    keep the CSG evaluation stacks in registers (their photons equal the oracle's in
+    test_production_kernels_bitexact); a tree of height 3 keeps the private-memory stacks."""
+    want = {'instanced_csg': (2, 1), 'thin_shells': (1, 1), 'csi_calorimeter': (0, 1), 'nested_boxes': (0, 1)}
+    for name, (h, reg) in want.items():
+        s = scene(name)
+        with engine_mod.Engine(s.geometry, s.physics, abi.default_config(**s.config)) as e:
+            i = e.info()
+        assert (i['csg_height'], i['csg_in_registers']) == (h, reg), name
+    f = Foundry()
+    f.add_material('Air', 1.0)
+    f.add_material('Glass', 1.5)
+    b0 = f.boundary('Air', None, None, 'Air')
+    b1 = f.boundary('Air', None, None, 'Glass')
+    f.begin_solid('r0')
+    f.add_prim(box3(1000.0), b0)
+    deep = ((sphere(50.0) | sphere(40.0, 30.0)) | (sphere(30.0, -30.0) | sphere(20.0, 0.0, 40.0))) | box3(20.0)
+    f.add_prim(deep, b1)
+    f.end_solid()
+    f.add_instance(np.eye(4), 0)
+    with engine_mod.Engine(f.arrays(), f.physics(), abi.default_config()) as e:
+        i = e.info()
+    assert (i['csg_height'], i['csg_in_registers']) == (3, 0)

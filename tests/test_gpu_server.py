@@ -1,5 +1,8 @@
 """Serving form: continuous batching keeps each event's result identical to running it
-alone, survives a failing request, and admits by the HBM budget."""
+alone and to the oracle, survives a failing request, and admits by the HBM budget. The
+serving workload's own stream (256 log-normal events on the calorimeter geometry) and
+events on the instanced-CSG geometry are served and checked event by event."""
+import threading
 import time
 
 import numpy as np
@@ -14,6 +17,42 @@ def bits(a):
     return np.ascontiguousarray(a).view(np.uint32)
 
 
+def oracle_hits(o, gs, event_index, hitmask, begin=0, end=-1):
+    """The oracle's hits of photons [begin, end) of one event, in photon order."""
+    r = o.simulate(gs, event_index=event_index, begin=begin, end=end, seq=False)
+    u = bits(r['photon'])
+    return u[(u[:, 3, 3] & hitmask) == hitmask]
+
+
+def serve(e, evs, event0, arrivals=None, **kw):
+    """Submit every event (from a feeder thread at the given arrival times, in seconds),
+    collect every result; returns ({request id: hits}, {request id: result}, stats)."""
+    from oxr.engine import Server
+    srv = Server(e, **kw)
+
+    def feed():
+        t0 = time.perf_counter()
+        for k, gs in enumerate(evs):
+            if arrivals is not None:
+                d = arrivals[k] - (time.perf_counter() - t0)
+                if d > 0:
+                    time.sleep(d)
+            srv.submit(k, gs, event0 + k)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    hits, res = {}, {}
+    for _ in evs:
+        r = srv.wait(120000)
+        assert r is not None and r['status'] == 0, r
+        hits[r['request_id']] = srv.take_hits(r['request_id'], r['num_hit'])
+        res[r['request_id']] = r
+    th.join()
+    st = srv.stats()
+    srv.close()
+    return hits, res, st
+
+
 def _events(s, rng, n, lo=500, hi=60000):
     out = []
     for k in range(n):
@@ -25,9 +64,10 @@ def _events(s, rng, n, lo=500, hi=60000):
 
 
 @pytest.mark.parametrize('lite', [0, 1])
-def test_alone_equals_in_mix(engine_mod, lite):
+def test_alone_equals_in_mix(engine_mod, oracle_lib, lite):
     """With photonlite on, single launches run the debug kernels (they write the lite
-    items) while batched steps run the production kernels: the hits still agree."""
+    items) while batched steps run the production kernels: the hits still agree, and
+    every served event's hits equal the oracle's hit selection of that event."""
     s = scenes.csi_calorimeter()
     cfg = abi.default_config(**s.config, photonlite=lite)
     rng = np.random.default_rng(12)
@@ -51,6 +91,10 @@ def test_alone_equals_in_mix(engine_mod, lite):
     for k in range(len(evs)):
         assert np.array_equal(bits(got[k]), bits(alone[k])), k
     assert st['steps'] >= 2 and st['photons'] == sum(G.num_photons(g) for g in evs)
+    o = oracle_lib.Oracle(s.geometry, s.physics, cfg)
+    o.set_cull(True)
+    for k, gs in enumerate(evs):
+        assert np.array_equal(bits(got[k]), oracle_hits(o, gs, 100 + k, cfg.hitmask)), k
 
 
 def test_failed_request_does_not_disturb_others(engine_mod):
@@ -97,3 +141,72 @@ def test_admission_budget_serialises_when_tight(engine_mod):
     rs.sort(key=lambda r: r['t_admit'])
     for a, b in zip(rs, rs[1:]):
         assert b['t_admit'] >= a['t_done'] - 1e-6
+
+
+def _check_served(s, cfg, evs, hits, res, alone, o, rng, full_below, sample):
+    """Per event: served hits == the event run alone; hits in photon order, each carrying
+    its own photon index, all flagged with the hit mask; photon totals conserved. Events
+    below `full_below` photons are replayed whole on the oracle, larger ones over one
+    random range of `sample` photons."""
+    for k, gs in enumerate(evs):
+        n = G.num_photons(gs)
+        h = bits(hits[k])
+        assert res[k]['num_photon'] == n and res[k]['num_hit'] == len(h)
+        assert np.array_equal(h, bits(alone[k])), k
+        idx = h[:, 3, 2].astype(np.int64)
+        assert np.all(np.diff(idx) > 0) and (len(idx) == 0 or (idx[0] >= 0 and idx[-1] < n)), k
+        assert np.all((h[:, 3, 3] & cfg.hitmask) == cfg.hitmask), k
+        if n <= full_below:
+            b, e_ = 0, n
+        else:
+            b = int(rng.integers(0, n - sample))
+            e_ = b + sample
+        want = oracle_hits(o, gs, 1000 + k, cfg.hitmask, b, e_)
+        got = h[(idx >= b) & (idx < e_)]
+        assert np.array_equal(got, want), (k, b, e_)
+
+
+def test_serving_stream_256_events(engine_mod, oracle_lib):
+    """The serving workload on the calorimeter geometry: 256 events, photons log-normal
+    (median 200k, sigma 1, clipped to 1k-20M), Poisson arrivals compressed 100x (40/s ->
+    4000/s), torch gensteps at random interior origins. Every event: served == alone,
+    ordered hits with their own indices, totals conserved; the oracle replays every event
+    below 20k photons whole and one random 10k-photon range of each larger one."""
+    s = scenes.csi_calorimeter()
+    cfg = abi.default_config(**s.config)
+    evs, counts, arrivals = scenes.serving_stream(np.random.default_rng(42))
+    assert len(evs) == 256 and counts.min() >= 1000 and counts.max() <= 20_000_000
+    print('serving stream:', int(counts.sum()), 'photons, largest event', int(counts.max()), flush=True)
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        hits, res, st = serve(e, evs, 1000, arrivals=arrivals / 100.0, max_step_photons=4 << 20)
+        print('served', int(st['steps']), 'steps', flush=True)
+        alone = [e.simulate(gs, event_index=1000 + k)[0] for k, gs in enumerate(evs)]
+    assert st['photons'] == int(counts.sum())
+    assert st['hits'] == sum(len(h) for h in hits.values())
+    o = oracle_lib.Oracle(s.geometry, s.physics, cfg)
+    o.set_cull(True)
+    _check_served(s, cfg, evs, hits, res, alone, o, np.random.default_rng(7), 20_000, 10_000)
+
+
+def test_serving_instanced_csg_events(engine_mod, oracle_lib):
+    """Served events on the instanced-CSG geometry (20,000 instanced 7-node CSG solids):
+    eight events of 20k-400k photons from random interior origins, batched into shared
+    persistent steps; each equal to its lone run and, over a sampled range, to the oracle."""
+    s = scenes.instanced_csg()
+    cfg = abi.default_config(**s.config)
+    rng = np.random.default_rng(5)
+    evs = []
+    for k in range(8):
+        m = int(rng.integers(20_000, 400_000))
+        ngs = max(1, m // 50_000)
+        u = rng.uniform(size=ngs) ** (1.0 / 3.0) * 17_000.0
+        v = rng.normal(size=(ngs, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        evs.append(G.torch(np.full(ngs, m // ngs), pos=v * u[:, None], wavelength=430.0, type=G.T_POINT_ISOTROPIC))
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        hits, res, st = serve(e, evs, 1000, max_step_photons=1 << 20)
+        alone = [e.simulate(gs, event_index=1000 + k)[0] for k, gs in enumerate(evs)]
+    assert st['steps'] >= 2 and st['photons'] == sum(G.num_photons(g) for g in evs)
+    o = oracle_lib.Oracle(s.geometry, s.physics, cfg)
+    o.set_cull(True)
+    _check_served(s, cfg, evs, hits, res, alone, o, np.random.default_rng(8), 0, 4000)

@@ -75,7 +75,9 @@ def test_simtrace_closed_form_on_sphere(oracle_lib):
 
 def test_simtrace_xyz_and_miss(oracle_lib):
     """A non-planar grid draws directions over the sphere; rays from outside every solid
-    miss: t 0, the hit position equals the start, identity words all ones."""
+    miss: the miss program's q0 (mid-grey background as normal, distance 1,
+    CSGOptiX7.cu:655-682 with SBT.cc:180-192), so the "hit" position is one step along the
+    direction from the start, and the identity words are all ones."""
     O, cfg = sphere_oracle(oracle_lib)
     gs = G.frame((0.0, 0.0, 0.0, 10.0), (1, 1, 1, 50))
     st = O.simtrace(gs)
@@ -83,6 +85,6 @@ def test_simtrace_xyz_and_miss(oracle_lib):
     assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6) and abs(d[:, 2].mean()) < 0.1
     far = G.frame((5000.0, 0.0, 0.0, 10.0), (0, 1, 1, 20))
     st = O.simtrace(far)
-    assert np.all(st[:, 0, 3] == 0.0)
-    assert np.array_equal(st[:, 1, :3], st[:, 2, :3])
+    assert np.all(st[:, 0, :3] == np.float32(0.6)) and np.all(st[:, 0, 3] == 1.0)
+    assert np.array_equal(st[:, 1, :3], st[:, 2, :3] + st[:, 3, :3])
     assert np.all(st[:, 2, 3].view(np.uint32) == 0xffffffff) and np.all(st[:, 3, 3].view(np.uint32) == 0xffffffff)
