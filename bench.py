@@ -301,12 +301,12 @@ def roofline_block(e, name, n, n_gs, n_hit, kms, generate_ms):
                                  '--only, read from the committed summary, not measured in this run') if prof else None)
 
 
-def setup(engine_mod, abi, scenes, name):
+def setup(engine_mod, abi, scenes, name, overrides=None):
     from oxr import gensteps as G
     s = scenes.SCENES[name]()
     cfg = abi.default_config(**s.config)
     e = engine_mod.Engine(s.geometry, s.physics, cfg)
-    tune = load_tuning(name)
+    tune = dict(load_tuning(name) or {}, **(overrides or {}))
     if tune:
         e.set_tuning(**tune)
     inp = None
@@ -322,9 +322,9 @@ def setup(engine_mod, abi, scenes, name):
     return s, cfg, e, gs, n, tune, inp
 
 
-def measure(engine_mod, abi, scenes, name, steps, warmup, roofline=True):
+def measure(engine_mod, abi, scenes, name, steps, warmup, roofline=True, overrides=None):
     """Persistent and plain forms of one workload, timed over `steps` launches each."""
-    s, cfg, e, gs, n, tune, inp = setup(engine_mod, abi, scenes, name)
+    s, cfg, e, gs, n, tune, inp = setup(engine_mod, abi, scenes, name, overrides)
     dt, st, kms = timed_launches(e, abi.OXR_FORM_PERSISTENT, steps, warmup)
     dtp, stp, kmsp = timed_launches(e, abi.OXR_FORM_PLAIN, steps, warmup)
     out = dict(workload=WORKLOADS[name], photons=n, gensteps=len(gs),
@@ -347,7 +347,9 @@ def measure(engine_mod, abi, scenes, name, steps, warmup, roofline=True):
 def only(engine_mod, abi, scenes, args):
     """The timed launches of one workload alone (persistent then plain), so that a kernel
     trace or counter run of this command holds only that workload's launches."""
-    out, _ = measure(engine_mod, abi, scenes, args.workload, args.steps, args.warmup, roofline=False)
+    overrides = {k: int(v) for k, v in (kv.split('=', 1) for kv in args.tune)}
+    out, _ = measure(engine_mod, abi, scenes, args.workload, args.steps, args.warmup, roofline=False,
+                     overrides=overrides)
     print(json.dumps(dict(metric=f'photons/s on 1 MI355X ({args.workload})', value=out['photons_per_s'],
                           unit='photons/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
                           ms_per_step=out['ms_per_step'], higher_is_better=True, scaling='weak',
@@ -367,6 +369,8 @@ def main():
     ap.add_argument('--no-precision', action='store_true', help='skip the thin-shell precision and history-chi2 legs')
     ap.add_argument('--workload', choices=sorted(WORKLOADS), default='instanced_csg')
     ap.add_argument('--only', action='store_true', help='time only the --workload launches (for profiling)')
+    ap.add_argument('--tune', action='append', default=[], metavar='KEY=VALUE',
+                    help='with --only: override a field of the committed tuning (e.g. csg_stack=1)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one MI355X: this engine runs one process on one GPU and the '

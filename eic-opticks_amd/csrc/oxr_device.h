@@ -647,14 +647,9 @@ OXD bool intersect_leaf(Is& is, const float4* node, int ni, const float4* plan, 
         default: break;
     }
     if (valid && gt > 0) {
-        // the transform rows are read again rather than held through the leaf: fewer
-        // registers live across the leaf arithmetic (the same values, from the cache)
-        asm volatile("" ::: "memory");
-        const float4* q = itra + 4 * (gt - 1);
-        const float4 r0 = q[0], r1 = q[1], r2 = q[2];
-        float x = r0.x * is.x + r0.y * is.y + r0.z * is.z;
-        float y = r1.x * is.x + r1.y * is.y + r1.z * is.z;
-        float z = r2.x * is.x + r2.y * is.y + r2.z * is.z;
+        float x = q0.x * is.x + q0.y * is.y + q0.z * is.z;
+        float y = q1.x * is.x + q1.y * is.y + q1.z * is.z;
+        float z = q2.x * is.x + q2.y * is.y + q2.z * is.z;
         is.x = x; is.y = y; is.z = z;
     }
     if (complement) {

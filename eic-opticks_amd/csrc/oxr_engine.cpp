@@ -70,6 +70,7 @@ oxr_tuning Engine::effective(const oxr_tuning& t) const {
     const bool tn = t.stack_mode == 1 && tn_stack_for(t, t.block_size);
     r.stack_mode = tn ? 1 : 0;
     r.lds_top_nodes = tn ? int(std::min<int64_t>(std::max(0, t.lds_top_nodes), top_nodes(t))) : ntop_for(t, t.block_size);
+    r.csg_stack = t.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
     return r;
 }
 
@@ -106,7 +107,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.ntop = ntop;
     P.stack_cap = stack_cap();
     P.tn_stack = tn ? 1 : 0;
-    P.shallow_csg = !has_list && csg_height <= 2 ? 1 : 0;
+    P.shallow_csg = tune.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
     P.refine = cfg.propagate_refine ? 1 : 0;
     P.refine_distance = cfg.refine_distance;
     P.refill_threshold = std::max(1, tune.refill_batch);
@@ -356,7 +357,8 @@ const char* oxr_last_error(void) { return g_err.c_str(); }
 static bool tuning_ok(const oxr_tuning& t) {
     return t.block_size >= 64 && t.block_size <= 256 && t.block_size % 64 == 0 && t.blocks_per_cu >= 1 &&
            t.blocks_per_cu <= 64 && t.lds_top_nodes >= 0 && t.refill_batch >= 1 && t.refill_batch <= 64 &&
-           t.stack_mode >= 0 && t.stack_mode <= 1 && t.bvh_mode >= 0 && t.bvh_mode <= 1;
+           t.stack_mode >= 0 && t.stack_mode <= 1 && t.bvh_mode >= 0 && t.bvh_mode <= 1 && t.csg_stack >= 0 &&
+           t.csg_stack <= 1;
 }
 
 static bool config_ok(const oxr_config& c) {
@@ -419,6 +421,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tune.refill_batch = 16;
     e->tune.stack_mode = 1;
     e->tune.bvh_mode = 0;
+    e->tune.csg_stack = 0;
     auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
     if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
@@ -684,7 +687,7 @@ int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
     std::strncpy(info->device_name, e->name.c_str(), sizeof(info->device_name) - 1);
     std::strncpy(info->arch, e->arch.c_str(), sizeof(info->arch) - 1);
     info->csg_height = e->csg_height;
-    info->csg_in_registers = !e->has_list && e->csg_height <= 2 ? 1 : 0;
+    info->csg_in_registers = e->tune.csg_stack == 1 && e->registers_csg_ok() ? 1 : 0;
     return OXR_OK;
 }
 
@@ -1082,16 +1085,16 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         }
         ms = tot / repeat;
         if (results && nres < max_results) {
-            double* r = results + 7 * nres;
+            double* r = results + OXR_AUTOTUNE_COLUMNS * nres;
             r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch;
-            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = ms;
+            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = t.csg_stack; r[7] = ms;
         }
         nres++;
         if (ms < best_ms) { best_ms = ms; best = t; }
         return OXR_OK;
     };
-    // coordinate search: BVH form, launch shape, LDS staging, refill granularity, then the
-    // traversal stack form
+    // coordinate search: BVH form, launch shape, LDS staging, refill granularity, the
+    // traversal stack form, then the CSG stack depth
     if (e->has_flat) {
         for (int bm : {0, 1}) {
             oxr_tuning t = base; t.bvh_mode = bm;
@@ -1120,6 +1123,13 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
     for (int sm : {0, 1}) {
         oxr_tuning t = b3; t.stack_mode = sm;
         double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
+    }
+    if (e->registers_csg_ok()) {
+        oxr_tuning b4 = best;
+        for (int cs : {0, 1}) {
+            oxr_tuning t = b4; t.csg_stack = cs;
+            double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
+        }
     }
     e->tune = best;
     if (num_results) *num_results = std::min(nres, max_results);   // rows written to results

@@ -102,6 +102,8 @@ struct Engine {
     int num_bnd = 0;
     bool has_list = false;   // some prim holds a list node: launches use the LIST kernels
     int csg_height = 0;      // greatest CSG tree height over the prims (0: leaves and lists only)
+    // the register-resident CSG stacks can serve every tree (tuning csg_stack 1)
+    bool registers_csg_ok() const { return !has_list && csg_height >= 1 && csg_height <= 2; }
     bool launched = false;
     bool hitlite_ready = false;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, eg = nullptr;

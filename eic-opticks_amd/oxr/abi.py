@@ -10,6 +10,7 @@ OXR_FORM_PLAIN = 0
 OXR_FORM_PERSISTENT = 1
 OXR_RNG_PRODUCTION = 0
 OXR_RNG_ALIGNED = 1
+OXR_AUTOTUNE_COLUMNS = 8
 (OXR_GET_PHOTON, OXR_GET_HIT, OXR_GET_SEQ, OXR_GET_RECORD, OXR_GET_SEED, OXR_GET_GENSTEP,
  OXR_GET_PHOTONLITE, OXR_GET_HITLITE, OXR_GET_REC, OXR_GET_PRD) = range(10)
 
@@ -58,7 +59,7 @@ class Config(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lds_top_nodes", C.c_int32), ("refill_batch", C.c_int32), ("stack_mode", C.c_int32),
-                ("bvh_mode", C.c_int32)]
+                ("bvh_mode", C.c_int32), ("csg_stack", C.c_int32)]
 
 
 class LaunchStats(C.Structure):

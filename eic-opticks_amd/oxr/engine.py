@@ -277,7 +277,7 @@ class Engine:
         return isect, valid
 
     def autotune(self, repeat=2, max_results=256):
-        res = np.zeros((max_results, 7), dtype=np.float64)
+        res = np.zeros((max_results, abi.OXR_AUTOTUNE_COLUMNS), dtype=np.float64)
         n = C.c_int32()
         _check(lib().oxr_autotune(self.h, repeat, abi.ptr(res), max_results, C.byref(n)))
         return res[:min(n.value, max_results)], self.tuning()

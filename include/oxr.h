@@ -168,6 +168,10 @@ typedef struct oxr_tuning {
     int32_t bvh_mode;         /* 1: one BVH over every placed prim (world boxes), when the
                                  engine built it (at most 2^21 placed prims); 0: instance
                                  BVH over per-solid prim BVHs                            */
+    int32_t csg_stack;        /* CSG evaluation stack depth: 1 keeps 3 partial intersects
+                                 and 3 tranches in registers (production kernels, when every
+                                 CSG tree has height 1 or 2 and no prim holds a list node);
+                                 0 keeps the reference's 15 and 4 in private memory       */
 } oxr_tuning;
 
 /* ---- outputs ---------------------------------------------------------- */
@@ -198,7 +202,8 @@ typedef struct oxr_engine_info {
     char    arch[32];
     int32_t csg_height;       /* greatest CSG tree height over the prims (0: leaves, lists)  */
     int32_t csg_in_registers; /* 1: production launches keep the CSG evaluation stacks in
-                                 registers (every tree of height <= 2, no list nodes)        */
+                                 registers (tuning csg_stack 1 in force: every tree of height
+                                 1 or 2, no list nodes)                                       */
 } oxr_engine_info;
 
 typedef struct oxr_engine oxr_engine;
@@ -326,11 +331,13 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
 
 /* ---- autotuner ----------------------------------------------------------
  * Times the persistent form on the current event over a grid of oxr_tuning values
- * (repeat times each) and applies the fastest. results (max_results,7) f64 =
- * block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode, kernel_ms,
+ * (repeat times each) and applies the fastest. results (max_results,OXR_AUTOTUNE_COLUMNS)
+ * f64 = block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode,
+ * csg_stack, kernel_ms,
  * one row per distinct effective tuning launched (see oxr_engine_set_tuning);
  * *num_results = rows written (at most max_results). On an error the tuning in force
  * before the call is restored. No reference counterpart. */
+#define OXR_AUTOTUNE_COLUMNS 8
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 

@@ -420,15 +420,19 @@ def test_debug_arrays_enabled_after_gensteps(engine_mod, oracle_lib):
 
 
 def test_csg_stacks_in_registers_for_shallow_trees(engine_mod):
-    """Geometries whose CSG trees are at most two levels deep (the instanced-CSG solid,
-    height 2; the thin-shell solid, height 1; leaf-only scenes) run production kernels that
-    keep the CSG evaluation stacks in registers (their photons equal the oracle's in
-    test_production_kernels_bitexact); a tree of height 3 keeps the private-memory stacks."""
-    want = {'instanced_csg': (2, 1), 'thin_shells': (1, 1), 'csi_calorimeter': (0, 1), 'nested_boxes': (0, 1)}
+    """Tuning csg_stack 1 keeps the CSG evaluation stacks in registers when every tree is
+    one or two levels deep (the instanced-CSG solid, height 2; the thin-shell solid, height
+    1); leaf-only scenes (height 0) and a tree of height 3 keep the private-memory stacks,
+    and so does csg_stack 0. Photons of both settings equal the oracle's in
+    test_gpu_tuning_forms."""
+    want = {'instanced_csg': (2, 1), 'thin_shells': (1, 1), 'csi_calorimeter': (0, 0), 'nested_boxes': (0, 0)}
     for name, (h, reg) in want.items():
         s = scene(name)
         with engine_mod.Engine(s.geometry, s.physics, abi.default_config(**s.config)) as e:
+            assert e.info()['csg_in_registers'] == 0 and e.tuning()['csg_stack'] == 0
+            e.set_tuning(csg_stack=1)
             i = e.info()
+            assert e.tuning()['csg_stack'] == reg, name
         assert (i['csg_height'], i['csg_in_registers']) == (h, reg), name
     f = Foundry()
     f.add_material('Air', 1.0)
@@ -442,5 +446,6 @@ def test_csg_stacks_in_registers_for_shallow_trees(engine_mod):
     f.end_solid()
     f.add_instance(np.eye(4), 0)
     with engine_mod.Engine(f.arrays(), f.physics(), abi.default_config()) as e:
+        e.set_tuning(csg_stack=1)
         i = e.info()
     assert (i['csg_height'], i['csg_in_registers']) == (3, 0)

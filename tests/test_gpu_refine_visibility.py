@@ -45,13 +45,14 @@ def test_hidden_instances(engine_mod, oracle_lib, form, seq):
 @pytest.mark.parametrize('form', [abi.OXR_FORM_PLAIN, abi.OXR_FORM_PERSISTENT])
 @pytest.mark.parametrize('seq', [0, 1])
 def test_refine_thin_shells(engine_mod, oracle_lib, form, seq):
-    """PropagateRefine on the thin shells (radii 1-20 m, so most traces go past the 5 m
-    refine distance) and on instanced CSG: GPU == oracle photon by photon."""
-    for name, make, frac in (('thin_shells', lambda: scenes.thin_shells(num_inst=300), 0.002),
-                             ('instanced_csg', lambda: scenes.instanced_csg(num_inst=2000), 0.002)):
+    """PropagateRefine on the thin shells (shells 0.39 m apart, so a 300 mm refine distance
+    is exceeded by most traces) and on instanced CSG (the default 5 m, exceeded by most
+    traces across the 19.5 m sphere): GPU == oracle photon by photon."""
+    for name, make, frac, dist in (('thin_shells', lambda: scenes.thin_shells(num_inst=300), 0.002, 300.0),
+                                   ('instanced_csg', lambda: scenes.instanced_csg(num_inst=2000), 0.002, 5000.0)):
         s = make()
         gs = s.gensteps(scale_photons=frac)
-        cfg = abi.default_config(**s.config, propagate_refine=1, seq_enabled=seq)
+        cfg = abi.default_config(**s.config, propagate_refine=1, refine_distance=dist, seq_enabled=seq)
         print('refine', name, 'form', form, 'seq', seq, flush=True)
         p, q, h, st = run(engine_mod, s, gs, cfg, form)
         o = oracle_lib.Oracle(s.geometry, s.physics, cfg)

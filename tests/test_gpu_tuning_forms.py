@@ -1,6 +1,7 @@
 """Every traversal form the tuning can select (two-level or flattened BVH, plain or
-distance-carrying stack) and launch shape gives the same photons, bit for bit, and the
-same photons as the oracle."""
+distance-carrying stack, CSG stacks in registers or private memory) and launch shape
+(workgroup size, LDS staging, workgroups per CU, refill batch) gives the same photons,
+bit for bit, and the same photons as the oracle."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,9 @@ from test_foundry_io import mixed_scene_arrays
 
 pytestmark = pytest.mark.gpu
 
-FORMS = [dict(bvh_mode=b, stack_mode=s) for b in (0, 1) for s in (0, 1)]
+FORMS = [dict(bvh_mode=b, stack_mode=s, csg_stack=c) for b in (0, 1) for s in (0, 1) for c in (0, 1)]
+# (block size, LDS top nodes, workgroups per CU, refill batch)
+SHAPES = [(256, 64, 3, 8), (128, 512, 8, 1), (64, 16, 1, 64)]
 
 
 def cases():
@@ -35,8 +38,8 @@ def test_forms_agree(engine_mod, oracle_lib, case):
     with engine_mod.Engine(geometry, physics, cfg) as e:
         e.set_gensteps(gs)
         for form in FORMS:
-            for block, top in ((256, 64), (128, 512)):
-                e.set_tuning(block_size=block, lds_top_nodes=top, **form)
+            for block, top, bpc, refill in SHAPES:
+                e.set_tuning(block_size=block, lds_top_nodes=top, blocks_per_cu=bpc, refill_batch=refill, **form)
                 for kind in (abi.OXR_FORM_PERSISTENT, abi.OXR_FORM_PLAIN):
                     print(name, form, block, top, kind, flush=True)
                     e.launch(kind)
