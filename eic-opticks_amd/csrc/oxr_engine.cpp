@@ -415,6 +415,9 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->num_cu = prop.multiProcessorCount;
     e->name = prop.name;
     e->arch = prop.gcnArchName;
+    // this device's address of the no-alloc entry point's scratch array, looked up once per
+    // device; a failure here is reported by oxr_simulate_noalloc, not by the engine
+    if ((he = oxl::noalloc_setup(dev)) != hipSuccess) set_error(std::string("no-alloc scratch: ") + hipGetErrorString(he));
     e->tune.block_size = 256;
     e->tune.blocks_per_cu = 3;
     e->tune.lds_top_nodes = 64;

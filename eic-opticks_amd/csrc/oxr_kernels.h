@@ -93,6 +93,8 @@ OXR_SIM_EXTERN(false, true)
 OXR_SIM_EXTERN(true, false)
 OXR_SIM_EXTERN(true, true)
 #undef OXR_SIM_EXTERN
+// looks up the device address of the no-alloc scratch array for `device` (oxr_noalloc.hip)
+hipError_t noalloc_setup(int device);
 int compact_blocks(long long n);
 hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
                    long long* offs, float4* hit, long long hit_cap);

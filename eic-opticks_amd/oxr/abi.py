@@ -10,6 +10,10 @@ OXR_FORM_PLAIN = 0
 OXR_FORM_PERSISTENT = 1
 OXR_RNG_PRODUCTION = 0
 OXR_RNG_ALIGNED = 1
+# oxr_simulate_noalloc limits (include/oxr.h)
+OXR_NOALLOC_MAX_PHOTON = 1 << 19
+OXR_NOALLOC_MAX_GENSTEP = 1 << 14
+OXR_NOALLOC_SCRATCH_BYTES = 12716288
 OXR_AUTOTUNE_COLUMNS = 8
 (OXR_GET_PHOTON, OXR_GET_HIT, OXR_GET_SEQ, OXR_GET_RECORD, OXR_GET_SEED, OXR_GET_GENSTEP,
  OXR_GET_PHOTONLITE, OXR_GET_HITLITE, OXR_GET_REC, OXR_GET_PRD) = range(10)

@@ -302,6 +302,26 @@ int oxr_simulate(oxr_engine* e, const float* gs, int64_t num_gs, int32_t event_i
                  int32_t form, void* hits, int64_t hit_capacity, int64_t* num_hit,
                  oxr_launch_stats* stats);
 
+/* One simulate launch that allocates no device memory. No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Host gensteps in (at most OXR_NOALLOC_MAX_GENSTEP,
+ * at most OXR_NOALLOC_MAX_PHOTON photons in total, no INPUT_PHOTON gensteps); the caller's
+ * device arrays out: d_photon (64 B per photon, in photon order) and d_hit (64 B per hit,
+ * up to hit_capacity, the hits in photon order); *num_hit is the hit count
+ * (OXR_ECAPACITY when it exceeds hit_capacity). Photon indices run from
+ * photon_slot_offset, as in a slice of oxr_simulate. Production kernels only: the
+ * config's seqhis, record, rec, prd and photonlite outputs are not written. Everything
+ * else the launch needs lives in a library-owned device array of
+ * OXR_NOALLOC_SCRATCH_BYTES on each GPU, held until the process ends and shared by
+ * every engine on that GPU; calls within a process are serialised by a lock. Larger
+ * events are refused with OXR_ECAPACITY. */
+#define OXR_NOALLOC_MAX_PHOTON  (1 << 19)
+#define OXR_NOALLOC_MAX_GENSTEP (1 << 14)
+#define OXR_NOALLOC_SCRATCH_BYTES 12716288
+int oxr_simulate_noalloc(oxr_engine* e, const float* gs, int64_t num_gs, int32_t event_index,
+                         uint64_t photon_slot_offset, int32_t form, float* d_photon, float* d_hit,
+                         int64_t hit_capacity, int64_t* num_hit, oxr_launch_stats* stats);
+
 /* ---- geometry query ----------------------------------------------------
  * Closest hit through the two-level BVH for arbitrary rays: the trace() contract of
  * CSGOptiX7.cu:110-216 with __intersection__is/__closesthit__ch/__miss__ms

@@ -74,6 +74,13 @@ def test_python_constants_match_header():
     assert {'OXR_GET_REC', 'OXR_GET_PRD', 'OXR_FORM_PERSISTENT', 'OXR_RNG_ALIGNED', 'OXR_OK'} <= set(shared)
     for k in shared:
         assert getattr(abi, k) == vals[k], k
+    # the #define constants (no-alloc limits, autotuner result columns)
+    defs = {m.group(1): eval(m.group(2)) for m in re.finditer(r'#define\s+(OXR_[A-Z0-9_]+)\s+([0-9()<x ]+)\n', hdr)}
+    assert {'OXR_NOALLOC_MAX_PHOTON', 'OXR_NOALLOC_MAX_GENSTEP', 'OXR_NOALLOC_SCRATCH_BYTES',
+            'OXR_AUTOTUNE_COLUMNS'} <= set(defs)
+    for k, v in defs.items():
+        if k.startswith(('OXR_NOALLOC_', 'OXR_AUTOTUNE_')):
+            assert getattr(abi, k) == v, k
 
 
 def test_tuning_range_checked_on_the_host(engine_mod):
