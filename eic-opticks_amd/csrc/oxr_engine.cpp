@@ -361,8 +361,21 @@ static bool tuning_ok(const oxr_tuning& t) {
            t.csg_stack <= 1;
 }
 
+// SEventConfig::HeuristicMaxSlot (sysrap/SEventConfig.cc:1897-1919): with max_photon 0 a
+// launch slice takes as many photons as fit in 87% of the device memory at this config's
+// per-photon bytes (photon, genstep index, resume state, hit flag, a hit, and the debug
+// arrays the config enables), rounded down to a million.
+static int64_t heuristic_max_photon(const oxr_config& c, size_t total_bytes) {
+    double per = 64.0 + 4.0 + 16.0 + 1.0 + 64.0;
+    if (c.seq_enabled) per += 32.0;
+    if (c.photonlite) per += 32.0;
+    per += 64.0 * c.max_record + 16.0 * c.max_rec + 32.0 * c.max_prd;
+    const int64_t n = int64_t(0.87 * double(total_bytes) / per / 1e6) * 1000000;
+    return std::max<int64_t>(n, 1000000);
+}
+
 static bool config_ok(const oxr_config& c) {
-    if (c.max_record < 0 || c.max_photon < 1 || c.max_rec < 0 || c.max_prd < 0) return false;
+    if (c.max_record < 0 || c.max_photon < 0 || c.max_rec < 0 || c.max_prd < 0) return false;
     // the rec domains divide each compressed coordinate
     if (c.max_rec > 0 && !(c.extent_domain > 0.f && c.time_domain > 0.f)) return false;
     if (c.propagate_refine > 1u || !std::isfinite(c.refine_distance)) return false;
@@ -380,7 +393,7 @@ void oxr_config_defaults(oxr_config* c) {
     c->rng_seed = 0; c->rng_offset = 0; c->skipahead_event = 100000;
     c->rng_schedule = OXR_RNG_PRODUCTION;
     c->fp64_roots = 0; c->seq_enabled = 0; c->max_record = 0;
-    c->max_photon = int64_t(1) << 24;
+    c->max_photon = 0;   // MaxSlot 0: sized from the device memory (SEventConfig.cc:90,1838-1852)
     c->photonlite = 0;
     c->max_rec = 0; c->max_prd = 0;
     c->extent_domain = 1000.f; c->time_domain = 10.f;   // SEventConfig.cc:72-73
@@ -413,6 +426,8 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     hipError_t he = hipGetDeviceProperties(&prop, dev);
     if (he != hipSuccess) { delete e; return hip_fail(he, "hipGetDeviceProperties"); }
     e->num_cu = prop.multiProcessorCount;
+    e->total_mem = prop.totalGlobalMem;
+    if (e->cfg.max_photon == 0) e->cfg.max_photon = heuristic_max_photon(e->cfg, e->total_mem);
     e->name = prop.name;
     e->arch = prop.gcnArchName;
     // this device's address of the no-alloc entry point's scratch array, looked up once per
@@ -646,6 +661,7 @@ int oxr_engine_set_config(oxr_engine* eh, const oxr_config* c) {
     if (!config_ok(*c)) return fail(OXR_EINVAL, "config");
     std::lock_guard<std::mutex> lk(e->mu);
     e->cfg = *c;
+    if (e->cfg.max_photon == 0) e->cfg.max_photon = heuristic_max_photon(e->cfg, e->total_mem);
     e->launched = false;
     return OXR_OK;
 }

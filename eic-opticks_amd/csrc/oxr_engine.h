@@ -62,6 +62,7 @@ struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cu = 0;
+    size_t total_mem = 0;   // device memory in bytes
     std::string name, arch;
     oxr_config cfg{};
     oxr_tuning tune{};

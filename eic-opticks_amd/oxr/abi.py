@@ -151,7 +151,7 @@ def default_config(**kw):
     c.fp64_roots = 0
     c.seq_enabled = 0
     c.max_record = 0
-    c.max_photon = 1 << 24
+    c.max_photon = 0            # MaxSlot 0: sized from the device memory
     c.photonlite = 0
     c.max_rec = 0
     c.max_prd = 0

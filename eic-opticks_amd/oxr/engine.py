@@ -126,6 +126,12 @@ class Engine:
         self.close()
 
     # -- config -----------------------------------------------------------
+    def config(self):
+        """The engine's config as in force (max_photon 0 resolved from the device memory)."""
+        c = abi.Config()
+        _check(lib().oxr_engine_get_config(self.h, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in c._fields_}
+
     def set_config(self, **kw):
         for k, v in kw.items():
             setattr(self.cfg, k, v)

@@ -135,7 +135,11 @@ typedef struct oxr_config {
                                  cylinder, cone), csg_robust_quadratic_roots.h:95-134 */
     int32_t  seq_enabled;     /* 1: write sseq seqhis/seqbnd per photon               */
     int32_t  max_record;      /* >0: write per-bounce sphoton records                 */
-    int64_t  max_photon;      /* capacity of one launch slice (OPTICKS_MAX_SLOT)      */
+    int64_t  max_photon;      /* capacity of one launch slice (OPTICKS_MAX_SLOT); 0, the
+                                 default, sizes it from the device memory as
+                                 SEventConfig::HeuristicMaxSlot (SEventConfig.cc:1897-1919):
+                                 87% of it over the config's bytes per photon, rounded
+                                 down to a million; oxr_engine_get_config returns it   */
     int32_t  photonlite;      /* 1: also write sphotonlite (16 B) per photon, the last
                                  intersect's lposcost/lposfphi packed (SEventConfig
                                  ModeLite, CSGOptiX7.cu:493-499); default 0           */

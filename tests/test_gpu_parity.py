@@ -449,3 +449,22 @@ def test_csg_stacks_in_registers_for_shallow_trees(engine_mod):
         e.set_tuning(csg_stack=1)
         i = e.info()
     assert (i['csg_height'], i['csg_in_registers']) == (3, 0)
+
+
+def test_max_photon_from_device_memory(engine_mod):
+    """max_photon 0 (the default, as the reference's MaxSlot 0) sizes a launch slice from
+    the device memory: 87% of it over the bytes per photon, rounded down to a million
+    (SEventConfig::HeuristicMaxSlot); debug arrays shrink it; an explicit value is kept."""
+    s = scene('nested_boxes')
+    with engine_mod.Engine(s.geometry, s.physics, abi.default_config(**s.config)) as e:
+        total = e.info()['total_bytes']
+        got = e.config()['max_photon']
+        e.set_config(seq_enabled=1, max_record=10, max_photon=0)
+        dbg = e.config()['max_photon']
+        e.set_config(max_photon=1500)
+        fixed = e.config()['max_photon']
+    # the engine sizes from the device properties' memory; the query reports hipMemGetInfo's
+    # total, which may differ by a little
+    assert got % 1000000 == 0 and abs(got - 0.87 * total / 149) < 2e6 and got > 1_000_000_000
+    assert dbg % 1000000 == 0 and abs(dbg - 0.87 * total / (149 + 32 + 640)) < 2e6 and dbg < got
+    assert fixed == 1500
