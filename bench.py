@@ -121,7 +121,7 @@ def cpu_baseline(scene, gs, cfg, threads, budget_s=12.0, inp=None):
     o = O.Oracle(scene.geometry, scene.physics, cfg)
     o.set_cull(True)
     n_total = int(np.asarray(gs).view(np.uint32)[:, 0, 3].astype(np.int64).sum())
-    probe = min(max(2000, 64 * threads), n_total)
+    probe = min(max(20000, 1000 * threads), n_total)
     t0 = time.perf_counter()
     o.simulate(gs, nthreads=threads, begin=0, end=probe, seq=False, input_photons=inp)
     rate = probe / max(time.perf_counter() - t0, 1e-6)
@@ -169,9 +169,9 @@ def history_chi2_leg(engine_mod, scene, gs, cfg, sample, threads, tune=None, inp
 
 
 def precision_leg(engine_mod, threads, sample=5000):
-    """Thin shells, 5M photons (10% grazing input photons): FP32 vs FP32+FP64 roots
-    throughput on the GPU, and both against the FP64 oracle on the same streams over a
-    sample of grazing and isotropic photons."""
+    """Thin shells, 5M photons (10% grazing input photons): FP32, FP32 with FP64 roots and
+    FP32 with the refine trace, throughput on the GPU, and each against the FP64 oracle on
+    the same streams over a sample of grazing and isotropic photons."""
     import oracle as O
     from oxr import abi, qcf, scenes, gensteps as G
     s = scenes.thin_shells()
@@ -187,8 +187,8 @@ def precision_leg(engine_mod, threads, sample=5000):
     rg = o.simulate(gs, precision=1, nthreads=threads, begin=0, end=sample, input_photons=inp)
     ri = o.simulate(gs, precision=1, nthreads=threads, begin=ng, end=ng + sample, input_photons=inp)
     oracle_s = time.perf_counter() - t0
-    for f64 in (0, 1):
-        cfg = abi.default_config(**s.config, fp64_roots=f64)
+    for f64, refine in ((0, 0), (1, 0), (0, 1)):
+        cfg = abi.default_config(**s.config, fp64_roots=f64, propagate_refine=refine)
         e = engine_mod.Engine(s.geometry, s.physics, cfg)
         if tune:
             e.set_tuning(**tune)
@@ -202,12 +202,14 @@ def precision_leg(engine_mod, threads, sample=5000):
         e.close()
         gra = qcf.endpoint_errors(ph[:sample], sq[:sample], rg['photon'], rg['seq'])
         iso = qcf.endpoint_errors(ph[ng:ng + sample], sq[ng:ng + sample], ri['photon'], ri['seq'])
-        rows['fp64_roots' if f64 else 'fp32'] = {
+        rows['fp64_roots' if f64 else ('fp32_refine' if refine else 'fp32')] = {
             'photons_per_s': round(3 * n / dt, 1), 'kernel_ms': round(kms, 3),
             'bounce_steps': int(st['bounce_steps']),
             'grazing_vs_fp64': {k: (round(v, 6) if isinstance(v, float) else v) for k, v in gra.items()},
             'isotropic_vs_fp64': {k: (round(v, 6) if isinstance(v, float) else v) for k, v in iso.items()}}
     rows['fp64_roots_cost'] = round(rows['fp32']['photons_per_s'] / rows['fp64_roots']['photons_per_s'], 4)
+    rows['refine_cost'] = round(rows['fp32']['photons_per_s'] / rows['fp32_refine']['photons_per_s'], 4)
+    rows['refine'] = 'fp32_refine: PropagateRefine on, refine distance 5000 mm (traces longer than 5.05 m re-traced from 0.99 t)' 
     rows['workload'] = ('thin_shells: 50 x 1 mm shells at 1-20 m (n 1.50/1.33) + 5000 instanced zsphere|cylinder; '
                         f'{n_total} photons at 400 nm, {ng} grazing (within 1e-3 rad of tangent) input photons '
                         'then isotropic torch from the origin; max_bounce 64')
