@@ -465,7 +465,9 @@ def main():
             'served_p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 1),
             'per_event_p50_ms': round(1e3 * float(np.percentile(lat_b, 50)), 1),
             'per_event_p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 1),
-            'served_steps': int(sst['steps'])}
+            'served_steps': int(sst['steps']),
+            'served_kernel_ms': round(sst['kernel_ms'], 2), 'served_wall_ms': round(1e3 * wall_s, 2),
+            'served_device_busy_frac': round(sst['kernel_ms'] / (1e3 * wall_s), 3)}
         log(f"[bench] served instanced_csg: {out['served_instanced_csg']}")
 
     if not args.no_precision:

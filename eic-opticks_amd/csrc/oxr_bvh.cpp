@@ -71,8 +71,14 @@ struct Builder {
             }
         }
         int mid;
-        if (best_axis < 0 || depth >= 24) {
-            // coincident centroids, or a deep branch: spatial median on the widest axis
+        if (best_axis < 0) {
+            // coincident centroids (concentric items): median by box size, so nested
+            // surfaces group by radius
+            mid = begin + n / 2;
+            std::nth_element(idx.begin() + begin, idx.begin() + mid, idx.begin() + end,
+                             [&](int p, int q) { return boxes[p].area() < boxes[q].area(); });
+        } else if (depth >= 24) {
+            // a deep branch: spatial median on the widest axis
             int ax = 0;
             for (int k = 1; k < 3; k++) if (cb.extent(k) > cb.extent(ax)) ax = k;
             mid = begin + n / 2;
@@ -106,9 +112,40 @@ void put_box(const Box& b, float* out6, double slack) {
     }
 }
 
+// Radial bound of a subtree about its own centre, merged bottom-up: a sphere item is
+// (its centre, r, r), any other item (box centre, 0, half diagonal); a parent takes its
+// box centre and the children's ranges shifted by their centres' distance from it (the
+// triangle inequality keeps the ranges valid, and exact for shared centres).
+struct Rad { double c[3]; double rin, rout; };
+
+double dist3(const double* a, const double* b) {
+    double x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+    return std::sqrt(x * x + y * y + z * z);
+}
+
+Rad item_rad(const Box& b, const RadItem& it) {
+    Rad r;
+    if (it.sphere) {
+        for (int k = 0; k < 3; k++) r.c[k] = it.c[k];
+        r.rin = r.rout = it.r;
+    } else {
+        for (int k = 0; k < 3; k++) r.c[k] = b.centroid(k);
+        r.rin = 0.0;
+        r.rout = 0.5 * std::sqrt(b.extent(0) * b.extent(0) + b.extent(1) * b.extent(1) + b.extent(2) * b.extent(2));
+    }
+    return r;
+}
+
+// the child's range about the parent's centre c
+void shifted(const Rad& ch, const double* c, double& rin, double& rout) {
+    double d = dist3(ch.c, c);
+    rin = std::max(0.0, ch.rin - d);
+    rout = ch.rout + d;
+}
+
 } // namespace
 
-Bvh build_bvh(const std::vector<Box>& boxes, int node_base, double slack) {
+Bvh build_bvh(const std::vector<Box>& boxes, int node_base, double slack, const std::vector<RadItem>* rad) {
     Bvh out;
     int n = int(boxes.size());
     if (n == 0) { out.root_ref = int32_t(0x80000000u); return out; }
@@ -144,6 +181,43 @@ Bvh build_bvh(const std::vector<Box>& boxes, int node_base, double slack) {
         o.c[0] = b1[2]; o.c[1] = b1[3]; o.c[2] = b1[4]; o.c[3] = b1[5];
         auto ref = [&](int c) { return B.tmp[c].item >= 0 ? int32_t(~B.tmp[c].item) : int32_t(node_base + bfs_index[c]); };
         o.ch[0] = ref(t.left); o.ch[1] = ref(t.right); o.ch[2] = 0; o.ch[3] = 0;
+    }
+    if (rad) {
+        // bottom-up over the temporary tree (children are created after their parent)
+        std::vector<Rad> R(B.tmp.size());
+        for (int t = int(B.tmp.size()) - 1; t >= 0; t--) {
+            const TmpNode& tn = B.tmp[size_t(t)];
+            if (tn.item >= 0) { R[size_t(t)] = item_rad(boxes[size_t(tn.item)], (*rad)[size_t(tn.item)]); continue; }
+            Rad& r = R[size_t(t)];
+            for (int k = 0; k < 3; k++) r.c[k] = tn.box.centroid(k);
+            double i0, o0, i1, o1;
+            shifted(R[size_t(tn.left)], r.c, i0, o0);
+            shifted(R[size_t(tn.right)], r.c, i1, o1);
+            r.rin = std::min(i0, i1);
+            r.rout = std::max(o0, o1);
+        }
+        out.rad.resize(order.size());
+        for (size_t k = 0; k < order.size(); k++) {
+            const TmpNode& t = B.tmp[size_t(order[k])];
+            const Rad& r = R[size_t(order[k])];
+            RadOut& o = out.rad[k];
+            double scale = 0.0;
+            for (int j = 0; j < 3; j++) { o.c[j] = float(r.c[j]); scale = std::max(scale, std::fabs(r.c[j])); }
+            o.c[3] = 0.f;
+            double ci[3] = {double(o.c[0]), double(o.c[1]), double(o.c[2])};   // the centre the device uses
+            int j = 0;
+            for (int ch : {t.left, t.right}) {
+                double rin, rout;
+                shifted(R[size_t(ch)], ci, rin, rout);
+                // widened like the boxes: the float distance |o - c| and the leaves' float roots
+                // stay inside these bounds for any ray origin in the scene
+                double m = 1e-5 * (scale + rout) + 0.05 + slack;
+                o.r[2 * j] = std::nextafter(float(std::max(0.0, rin - m)), -std::numeric_limits<float>::infinity());
+                if (o.r[2 * j] < 0.f) o.r[2 * j] = 0.f;
+                o.r[2 * j + 1] = std::nextafter(float(rout + m), std::numeric_limits<float>::infinity());
+                j++;
+            }
+        }
     }
     out.root_ref = node_base;
     return out;

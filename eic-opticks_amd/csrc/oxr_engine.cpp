@@ -38,6 +38,7 @@ oxd::Geom Engine::geom() const {
     g.flat = f ? flat.p : nullptr;
     g.tlas_root = f ? tlas_root_f : tlas_root; g.num_inst = num_inst;
     g.vizmask = cfg.vizmask;
+    g.rad_top = f ? rad_flat.p : rad_tlas.p; g.rad_blas = rad_blas.p;
     return g;
 }
 
@@ -108,6 +109,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.stack_cap = stack_cap();
     P.tn_stack = tn ? 1 : 0;
     P.shallow_csg = tune.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
+    P.radial = tune.bvh_radial == 1 ? 1 : 0;
     P.refine = cfg.propagate_refine ? 1 : 0;
     P.refine_distance = cfg.refine_distance;
     P.refill_threshold = std::max(1, tune.refill_batch);
@@ -358,7 +360,7 @@ static bool tuning_ok(const oxr_tuning& t) {
     return t.block_size >= 64 && t.block_size <= 256 && t.block_size % 64 == 0 && t.blocks_per_cu >= 1 &&
            t.blocks_per_cu <= 64 && t.lds_top_nodes >= 0 && t.refill_batch >= 1 && t.refill_batch <= 64 &&
            t.stack_mode >= 0 && t.stack_mode <= 1 && t.bvh_mode >= 0 && t.bvh_mode <= 1 && t.csg_stack >= 0 &&
-           t.csg_stack <= 1;
+           t.csg_stack <= 1 && t.bvh_radial >= 0 && t.bvh_radial <= 1;
 }
 
 // SEventConfig::HeuristicMaxSlot (sysrap/SEventConfig.cc:1897-1919): with max_photon 0 a
@@ -440,6 +442,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tune.stack_mode = 1;
     e->tune.bvh_mode = 0;
     e->tune.csg_stack = 0;
+    e->tune.bvh_radial = 0;
     auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
     if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
@@ -456,18 +459,23 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
 
     // ---- per-solid BLAS over prim AABBs
     std::vector<oxh::NodeOut> blas_all;
+    std::vector<oxh::RadOut> rad_blas;
     int max_blas_depth = 0;
     e->solid_prim_offset.resize(size_t(g->num_solid));
     e->solid_num_prim.resize(size_t(g->num_solid));
     e->solid_blas_root.resize(size_t(g->num_solid));
     std::vector<oxh::Box> solid_box(static_cast<size_t>(g->num_solid));
     std::vector<std::vector<oxh::Box>> prim_boxes(static_cast<size_t>(g->num_solid));
+    // a prim whose root node is a sphere leaf without a transform has a known surface: the
+    // sphere (centre, radius in node q0); any other prim lies somewhere in its box
+    std::vector<std::vector<oxh::RadItem>> prim_rad(static_cast<size_t>(g->num_solid));
     for (int64_t s = 0; s < g->num_solid; s++) {
         const int32_t* so = g->solid + 12 * s;
         int numPrim = so[4], primOffset = so[5];
         if (numPrim < 1 || primOffset < 0 || int64_t(primOffset) + numPrim > g->num_prim)
             return bail(fail(OXR_EINVAL, "solid " + std::to_string(s) + ": prim range outside prim array"));
         std::vector<oxh::Box> boxes(static_cast<size_t>(numPrim));
+        std::vector<oxh::RadItem> rads(static_cast<size_t>(numPrim));
         oxh::Box sb = oxh::Box::empty();
         for (int p = 0; p < numPrim; p++) {
             const float* q = g->prim + 16 * size_t(primOffset + p);
@@ -477,7 +485,17 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
             for (int k = 0; k < 3; k++) if (!(b.lo[k] <= b.hi[k])) return bail(fail(OXR_EINVAL, "prim with empty/NaN AABB"));
             boxes[size_t(p)] = b;
             sb.grow(b);
+            int32_t nodeOffset;
+            std::memcpy(&nodeOffset, q + 1, 4);
+            const float* nd = g->node + 16 * size_t(nodeOffset);
+            uint32_t tc, gt;
+            std::memcpy(&tc, nd + 14, 4);
+            std::memcpy(&gt, nd + 15, 4);
+            oxh::RadItem& ri = rads[size_t(p)];
+            ri.sphere = tc == 101u && (gt & 0x7fffffffu) == 0u && nd[3] > 0.f;
+            ri.c[0] = nd[0]; ri.c[1] = nd[1]; ri.c[2] = nd[2]; ri.r = nd[3];
         }
+        prim_rad[size_t(s)] = std::move(rads);
         e->solid_prim_offset[size_t(s)] = primOffset;
         e->solid_num_prim[size_t(s)] = numPrim;
         solid_box[size_t(s)] = sb;
@@ -509,10 +527,11 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     }
     const double slack_world = 1e-6 * 2.0 * W, slack_obj = 1e-6 * std::max(S, 2.0 * W);
     for (int64_t s = 0; s < g->num_solid; s++) {
-        oxh::Bvh bv = oxh::build_bvh(prim_boxes[size_t(s)], int(blas_all.size()), slack_obj);
+        oxh::Bvh bv = oxh::build_bvh(prim_boxes[size_t(s)], int(blas_all.size()), slack_obj, &prim_rad[size_t(s)]);
         e->solid_blas_root[size_t(s)] = bv.root_ref;
         max_blas_depth = std::max(max_blas_depth, bv.depth);
         blas_all.insert(blas_all.end(), bv.nodes.begin(), bv.nodes.end());
+        rad_blas.insert(rad_blas.end(), bv.rad.begin(), bv.rad.end());
     }
     // ---- instances and TLAS
     std::vector<oxd::InstD> insts(static_cast<size_t>(g->num_inst));
@@ -570,7 +589,8 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     std::vector<int2> flat_items;
     if (e->has_flat) {
         std::vector<oxh::Box> leaf_box;
-        leaf_box.reserve(size_t(placed)); flat_items.reserve(size_t(placed));
+        std::vector<oxh::RadItem> leaf_rad;
+        leaf_box.reserve(size_t(placed)); flat_items.reserve(size_t(placed)); leaf_rad.reserve(size_t(placed));
         for (int64_t i = 0; i < g->num_inst; i++) {
             const float* q = g->inst + 16 * i;
             int32_t gas; std::memcpy(&gas, q + 7, 4);
@@ -586,14 +606,22 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
                 }
                 leaf_box.push_back(wb);
                 flat_items.push_back(make_int2(int(i), int(p)));
+                // world-frame sphere only for an instance with the identity transform
+                oxh::RadItem ri = prim_rad[size_t(gas)][p];
+                const bool ident = q[0] == 1.f && q[1] == 0.f && q[2] == 0.f && q[4] == 0.f && q[5] == 1.f && q[6] == 0.f &&
+                                   q[8] == 0.f && q[9] == 0.f && q[10] == 1.f && q[12] == 0.f && q[13] == 0.f && q[14] == 0.f;
+                ri.sphere = ri.sphere && ident;
+                leaf_rad.push_back(ri);
             }
         }
-        tf = oxh::build_bvh(leaf_box, 0, slack_world);
+        tf = oxh::build_bvh(leaf_box, 0, slack_world, &leaf_rad);
         e->stack_need_f = tf.depth + 1;
         e->tlas_root_f = tf.root_ref;
         e->tlas_nodes_f = int64_t(tf.nodes.size());
     }
-    tl = oxh::build_bvh(inst_box, 0, slack_world);
+    std::vector<oxh::RadItem> inst_rad(inst_box.size());
+    for (auto& r : inst_rad) r.sphere = false;
+    tl = oxh::build_bvh(inst_box, 0, slack_world, &inst_rad);
     e->stack_need = tl.depth + 1 + max_blas_depth;
     e->tlas_root = tl.root_ref;
     e->tlas_nodes = int64_t(tl.nodes.size());
@@ -621,6 +649,9 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     if ((rc = up(e->flat, flat_items.data(), flat_items.size(), 8))) return bail(rc);
     if ((rc = up(e->tlas_f, tf.nodes.data(), tf.nodes.size(), 64))) return bail(rc);
     if ((rc = up(e->inst, insts.data(), insts.size(), 64))) return bail(rc);
+    if ((rc = up(e->rad_tlas, tl.rad.data(), tl.rad.size() * 2, 16))) return bail(rc);
+    if ((rc = up(e->rad_blas, rad_blas.data(), rad_blas.size() * 2, 16))) return bail(rc);
+    if ((rc = up(e->rad_flat, tf.rad.data(), tf.rad.size() * 2, 16))) return bail(rc);
     size_t bnd_f4 = size_t(ph->num_bnd) * 4 * 2 * size_t(ph->num_wl);
     if ((rc = up(e->bnd, ph->bnd, bnd_f4, 16))) return bail(rc);
     if ((rc = up(e->optical, ph->optical, size_t(ph->num_bnd) * 4, 16))) return bail(rc);
@@ -645,7 +676,8 @@ void oxr_engine_destroy(oxr_engine* eh) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     e->ev.release_all();
     e->node.release(); e->plan.release(); e->itra.release(); e->bnd.release(); e->prim.release();
-    e->tlas.release(); e->blas.release(); e->inst.release(); e->flat.release(); e->tlas_f.release(); e->optical.release(); e->icdf.release();
+    e->tlas.release(); e->blas.release(); e->inst.release(); e->flat.release(); e->tlas_f.release();
+    e->rad_tlas.release(); e->rad_blas.release(); e->rad_flat.release(); e->optical.release(); e->icdf.release();
     if (e->e0) (void)hipEventDestroy(e->e0);
     if (e->e1) (void)hipEventDestroy(e->e1);
     if (e->e2) (void)hipEventDestroy(e->e2);
@@ -1106,17 +1138,18 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         if (results && nres < max_results) {
             double* r = results + OXR_AUTOTUNE_COLUMNS * nres;
             r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch;
-            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = t.csg_stack; r[7] = ms;
+            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = t.csg_stack; r[7] = t.bvh_radial; r[8] = ms;
         }
         nres++;
         if (ms < best_ms) { best_ms = ms; best = t; }
         return OXR_OK;
     };
-    // coordinate search: BVH form, launch shape, LDS staging, refill granularity, the
-    // traversal stack form, then the CSG stack depth
-    if (e->has_flat) {
-        for (int bm : {0, 1}) {
-            oxr_tuning t = base; t.bvh_mode = bm;
+    // coordinate search: BVH form and radial culling, launch shape, LDS staging, refill
+    // granularity, the traversal stack form, then the CSG stack depth
+    for (int bm : {0, 1}) {
+        for (int br : {0, 1}) {
+            if (bm == 1 && !e->has_flat) continue;
+            oxr_tuning t = base; t.bvh_mode = bm; t.bvh_radial = br;
             double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
         }
     }

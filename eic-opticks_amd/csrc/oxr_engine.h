@@ -73,6 +73,7 @@ struct Engine {
     DBuf<oxd::InstD> inst;
     DBuf<int2> flat;             // (instance, local prim) per leaf of the flattened BVH
     DBuf<oxd::BNode> tlas_f;     // the flattened BVH (built when the placed prims are few enough)
+    DBuf<float4> rad_tlas, rad_blas, rad_flat;   // radial bounds, two float4 per node of each tree
     bool has_flat = false;
     int tlas_root_f = 0;
     int64_t tlas_nodes_f = 0;

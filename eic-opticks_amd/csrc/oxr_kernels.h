@@ -59,6 +59,7 @@ struct Params {
     int stack_cap;             // per-lane traversal stack entries in LDS
     int tn_stack;              // stack entries carry their box entry distance (8 B each)
     int shallow_csg;           // every CSG tree has height <= 2 and no prim holds a list node
+    int radial;                // nodes also cull by their radial bounds (tuning bvh_radial)
     int refine;                // PropagateRefine: trace again from 0.99 of a far hit
     float refine_distance;     // PropagateRefineDistance (mm)
     int refill_threshold;      // persistent form: idle lanes per wave before a refill
@@ -79,7 +80,8 @@ hipError_t simulate_plain(hipStream_t s, const oxd::Params& P, int block, bool f
 hipError_t simulate_persist(hipStream_t s, const oxd::Params& P, int block, int grid, bool fp64, bool list);
 // The simulate kernel variants of one (FP64 roots, list support) pair, defined in oxr_sim.h
 // and instantiated in oxr_sim_{f32,f64}[_list].hip. kind: SIM_PROD, SIM_DEBUG or SIM_COUNT;
-// tm: traversal mode, bit 0 distance-carrying stack, bit 1 flattened BVH.
+// tm: traversal mode, bit 0 distance-carrying stack, bit 1 flattened BVH, bit 2 register CSG
+// stacks, bit 3 radial culling.
 enum { SIM_PROD = 0, SIM_DEBUG = 1, SIM_COUNT = 2 };
 template <bool FP64, bool LIST>
 hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, int kind, int tm);

@@ -2,7 +2,8 @@
 // form), templated on root precision (FP64), counting (COUNT), debug outputs (DBG), list-node
 // support (LIST) and the traversal mode TM (bit 0: stack entries carry box entry distances;
 // bit 1: the flattened BVH; bit 2: every CSG tree has height <= 2, so the CSG evaluation
-// stacks live in registers; production kernels without list nodes only). k_generate (oxr_kernels.hip) uses the per-photon pieces; the
+// stacks live in registers, production kernels without list nodes only; bit 3: nodes also
+// cull by their radial bounds, production and counting kernels). k_generate (oxr_kernels.hip) uses the per-photon pieces; the
 // kernels are instantiated by the four oxr_sim_*.hip units, one per (FP64, LIST) pair, which
 // compile in parallel.
 #pragma once
@@ -109,7 +110,7 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
 // 0.99 part added back, so a far surface is intersected from close by. A miss reports
 // distance 1, as the miss program writes it (CSGOptiX7.cu:666-670), so h.t is the
 // reference's prd distance in every case. One trace call site serves both passes.
-template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW>
+template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW, bool RADIAL>
 OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int* stk, int stride,
                        const BNode* top, int ntop, uint32_t& visits, uint32_t& tests, uint32_t& err,
                        uint32_t& csgb, uint32_t& ivis) {
@@ -117,8 +118,8 @@ OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int*
     float t_approx = 0.f;
     bool refined = false, found;
     for (;;) {
-        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW>(P.g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap, top,
-                                                           ntop, visits, tests, err, csgb, ivis);
+        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW, RADIAL>(P.g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap,
+                                                                   top, ntop, visits, tests, err, csgb, ivis);
         if (!found) h.t = 1.f;
         if (!P.refine || refined) break;
         t_approx = 0.99f * h.t;
@@ -137,7 +138,7 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return false;
     float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Hit h;
-    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0>(
+    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0, (TM & 8) != 0>(
         P, L.p.pos, L.p.mom, tmin, h, stk, stride, top, ntop, visits, tests, err, csgb, ivis);
     if (!found) {
         if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0
@@ -303,38 +304,37 @@ __global__ void __launch_bounds__(256, OXR_PERSIST_WAVES) k_simulate_persist(Par
 namespace oxl {
 
 // kind: SIM_PROD (production), SIM_DEBUG (seqhis, records, precooked streams), SIM_COUNT
-// (visit counting, generating in place like the debug variants). Debug launches use the
-// node-index stack (tm bit 0 clear).
+// (visit counting, generating in place like the debug variants). tm: traversal mode bits
+// (oxd kernels above); debug launches use the node-index stack without radial culling,
+// counting launches the production traversal without the register CSG stacks (which
+// change no visit), so they count the visits the production launch makes.
 template <bool FP64, bool LIST>
 hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, int kind, int tm) {
 #define OXR_PLAIN(C, D, T) hipLaunchKernelGGL((oxd::k_simulate<FP64, C, D, LIST, T>), g, b, sm, s, P)
     if (kind == SIM_COUNT) {
-        switch (tm & 3) {
+        switch (tm & 11) {
             case 0: OXR_PLAIN(true, true, 0); break;
             case 1: OXR_PLAIN(true, true, 1); break;
             case 2: OXR_PLAIN(true, true, 2); break;
-            default: OXR_PLAIN(true, true, 3); break;
+            case 3: OXR_PLAIN(true, true, 3); break;
+            case 8: OXR_PLAIN(true, true, 8); break;
+            case 9: OXR_PLAIN(true, true, 9); break;
+            case 10: OXR_PLAIN(true, true, 10); break;
+            default: OXR_PLAIN(true, true, 11); break;
         }
     } else if (kind == SIM_DEBUG) {
         if (tm & 2) OXR_PLAIN(false, true, 2);
         else OXR_PLAIN(false, true, 0);
     } else {
-        switch (tm) {
-            case 0: OXR_PLAIN(false, false, 0); break;
-            case 1: OXR_PLAIN(false, false, 1); break;
-            case 2: OXR_PLAIN(false, false, 2); break;
-            case 3: OXR_PLAIN(false, false, 3); break;
-            default:
-                if constexpr (!LIST) {
-                    switch (tm) {
-                        case 4: OXR_PLAIN(false, false, 4); break;
-                        case 5: OXR_PLAIN(false, false, 5); break;
-                        case 6: OXR_PLAIN(false, false, 6); break;
-                        default: OXR_PLAIN(false, false, 7); break;
-                    }
-                } else {
-                    return hipErrorInvalidValue;
-                }
+        switch (tm & 11) {
+            case 0: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 4 * !LIST); else OXR_PLAIN(false, false, 0); break;
+            case 1: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 1 + 4 * !LIST); else OXR_PLAIN(false, false, 1); break;
+            case 2: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 2 + 4 * !LIST); else OXR_PLAIN(false, false, 2); break;
+            case 3: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 3 + 4 * !LIST); else OXR_PLAIN(false, false, 3); break;
+            case 8: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 8 + 4 * !LIST); else OXR_PLAIN(false, false, 8); break;
+            case 9: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 9 + 4 * !LIST); else OXR_PLAIN(false, false, 9); break;
+            case 10: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 10 + 4 * !LIST); else OXR_PLAIN(false, false, 10); break;
+            default: if ((tm & 4) && !LIST) OXR_PLAIN(false, false, 11 + 4 * !LIST); else OXR_PLAIN(false, false, 11); break;
         }
     }
 #undef OXR_PLAIN
@@ -348,22 +348,15 @@ hipError_t launch_simulate_persist(hipStream_t s, const oxd::Params& P, dim3 g, 
         if (tm & 2) OXR_PERSIST(true, 2);
         else OXR_PERSIST(true, 0);
     } else {
-        switch (tm) {
-            case 0: OXR_PERSIST(false, 0); break;
-            case 1: OXR_PERSIST(false, 1); break;
-            case 2: OXR_PERSIST(false, 2); break;
-            case 3: OXR_PERSIST(false, 3); break;
-            default:
-                if constexpr (!LIST) {
-                    switch (tm) {
-                        case 4: OXR_PERSIST(false, 4); break;
-                        case 5: OXR_PERSIST(false, 5); break;
-                        case 6: OXR_PERSIST(false, 6); break;
-                        default: OXR_PERSIST(false, 7); break;
-                    }
-                } else {
-                    return hipErrorInvalidValue;
-                }
+        switch (tm & 11) {
+            case 0: if ((tm & 4) && !LIST) OXR_PERSIST(false, 4 * !LIST); else OXR_PERSIST(false, 0); break;
+            case 1: if ((tm & 4) && !LIST) OXR_PERSIST(false, 1 + 4 * !LIST); else OXR_PERSIST(false, 1); break;
+            case 2: if ((tm & 4) && !LIST) OXR_PERSIST(false, 2 + 4 * !LIST); else OXR_PERSIST(false, 2); break;
+            case 3: if ((tm & 4) && !LIST) OXR_PERSIST(false, 3 + 4 * !LIST); else OXR_PERSIST(false, 3); break;
+            case 8: if ((tm & 4) && !LIST) OXR_PERSIST(false, 8 + 4 * !LIST); else OXR_PERSIST(false, 8); break;
+            case 9: if ((tm & 4) && !LIST) OXR_PERSIST(false, 9 + 4 * !LIST); else OXR_PERSIST(false, 9); break;
+            case 10: if ((tm & 4) && !LIST) OXR_PERSIST(false, 10 + 4 * !LIST); else OXR_PERSIST(false, 10); break;
+            default: if ((tm & 4) && !LIST) OXR_PERSIST(false, 11 + 4 * !LIST); else OXR_PERSIST(false, 11); break;
         }
     }
 #undef OXR_PERSIST

@@ -38,6 +38,10 @@ struct Geom {
     int tlas_root;
     int num_inst;
     unsigned vizmask;      // ray visibility mask against InstD flag bits 8-15
+    // radial bounds, two float4 per node {centre, (r_in0, r_out0, r_in1, r_out1)} of the
+    // top tree (instance tree or flattened tree) and of the per-solid trees
+    const float4* rad_top;
+    const float4* rad_blas;
 };
 
 constexpr int SENTINEL = int(0x80000000u);
@@ -83,6 +87,11 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 
 // Closest hit in (tmin, tmax]; ties in t go to the smaller (instance, local prim) key.
 // SHALLOW: every CSG tree of the geometry has height <= 2 (register-resident CSG stacks).
+// RADIAL: besides its box, each child must pass its radial bound: its surfaces lie between
+// r_in and r_out from the node's centre c, so along a ray from o (direction length |d|) they
+// are no nearer than max(r_in - |o - c|, |o - c| - r_out) / |d|; a child whose bound lies
+// beyond the best hit is skipped, and the bound orders the children and fills the distance
+// stack. Concentric shells, whose boxes all hold the ray origin, are culled this way.
 // stk points at this lane's LDS stack column (element k at stk[k*stride]). With TN,
 // a second column of cap floats follows the stack (at stk + stride*cap) holding each
 // pushed node's box entry distance, and a popped node whose entry lies beyond the best
@@ -91,7 +100,7 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 // During traversal only (t, instance, local prim, object-frame normal) of the best hit
 // stay live; its world normal, local position and identity words are formed afterwards
 // from the instance record, with the same arithmetic as at instance entry.
-template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false, bool SHALLOW = false>
+template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false, bool SHALLOW = false, bool RADIAL = false>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
@@ -103,6 +112,8 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
     int cur = g.tlas_root;
     bool at_top = true;
     f3 o = pos, d = mom, inv = safe_inv(mom), noi = neg_oi(pos, inv);
+    const float ilen_world = RADIAL ? __builtin_amdgcn_rsqf(dot(mom, mom)) : 1.f;
+    float ilen = ilen_world;
     float* stkt = reinterpret_cast<float*>(stk + stride * cap);
     // pop the next entry worth visiting; false when the stack is empty
     auto pop = [&]() -> bool {
@@ -123,6 +134,18 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             float tn0, tn1;
             bool h0 = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, noi, inv, best, tn0);
             bool h1 = box_hit(nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, noi, inv, best, tn1);
+            if (RADIAL) {
+                const float4* rb = at_top ? g.rad_top : g.rad_blas;
+                const float4 rc = rb[2 * cur], rr = rb[2 * cur + 1];
+                const float dx = o.x - rc.x, dy = o.y - rc.y, dz = o.z - rc.z;
+                const float dist = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz);
+                const float lb0 = fmaxf(rr.x - dist, dist - rr.y) * ilen;
+                const float lb1 = fmaxf(rr.z - dist, dist - rr.w) * ilen;
+                h0 = h0 && lb0 <= best;
+                h1 = h1 && lb1 <= best;
+                tn0 = fmaxf(tn0, lb0);
+                tn1 = fmaxf(tn1, lb1);
+            }
             if (h0 && h1) {
                 bool first0 = tn0 <= tn1;
                 int nearc = first0 ? nd.ch.x : nd.ch.y;
@@ -147,6 +170,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
         if (cur == SENTINEL) {
             at_top = true;
             o = pos; d = mom; inv = safe_inv(mom); noi = neg_oi(pos, inv);
+            ilen = ilen_world;
             if (!pop()) break;
             continue;
         }
@@ -200,6 +224,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                 to_object(in, pos, mom, o, d);
                 inv = safe_inv(d);
                 noi = neg_oi(o, inv);
+                if (RADIAL) ilen = __builtin_amdgcn_rsqf(dot(d, d));
             }
             if (sp < cap) {
                 stk[sp * stride] = SENTINEL;

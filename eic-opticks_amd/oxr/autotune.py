@@ -2,7 +2,8 @@
 
 Runs the C-ABI autotuner (coordinate search over workgroup size, workgroups per CU,
 top-level BVH nodes staged in LDS, refill granularity, the traversal stack form, the BVH
-form and the CSG evaluation stack depth, timed with HIP events) on a
+form with or without radial culling and the CSG evaluation stack depth, timed with HIP
+events) on a
 workload and stores the winning settings in tuning_gfx950.json, which bench.py and the
 serving loop apply.
 
@@ -34,7 +35,8 @@ def tune(name, repeat=2, scale=1.0, out=TABLE):
     table.setdefault('_trials', {})[name] = [dict(block_size=int(r[0]), blocks_per_cu=int(r[1]),
                                                   lds_top_nodes=int(r[2]), refill_batch=int(r[3]),
                                                   stack_mode=int(r[4]), bvh_mode=int(r[5]),
-                                                  csg_stack=int(r[6]), kernel_ms=round(float(r[7]), 4))
+                                                  csg_stack=int(r[6]), bvh_radial=int(r[7]),
+                                                  kernel_ms=round(float(r[8]), 4))
                                              for r in res]
     with open(out, 'w') as f:
         json.dump(table, f, indent=1, sort_keys=True)

@@ -176,6 +176,10 @@ typedef struct oxr_tuning {
                                  and 3 tranches in registers (production kernels, when every
                                  CSG tree has height 1 or 2 and no prim holds a list node);
                                  0 keeps the reference's 15 and 4 in private memory       */
+    int32_t bvh_radial;       /* 1: BVH nodes also cull by radial bounds (a centre and the
+                                 range of distances of each child's surfaces from it: exact
+                                 for sphere prims, box-based otherwise), which removes the
+                                 concentric shells whose boxes all hold the ray origin     */
 } oxr_tuning;
 
 /* ---- outputs ---------------------------------------------------------- */
@@ -357,11 +361,11 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
  * Times the persistent form on the current event over a grid of oxr_tuning values
  * (repeat times each) and applies the fastest. results (max_results,OXR_AUTOTUNE_COLUMNS)
  * f64 = block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode,
- * csg_stack, kernel_ms,
+ * csg_stack, bvh_radial, kernel_ms,
  * one row per distinct effective tuning launched (see oxr_engine_set_tuning);
  * *num_results = rows written (at most max_results). On an error the tuning in force
  * before the call is restored. No reference counterpart. */
-#define OXR_AUTOTUNE_COLUMNS 8
+#define OXR_AUTOTUNE_COLUMNS 9
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 

@@ -11,7 +11,8 @@ from test_foundry_io import mixed_scene_arrays
 
 pytestmark = pytest.mark.gpu
 
-FORMS = [dict(bvh_mode=b, stack_mode=s, csg_stack=c) for b in (0, 1) for s in (0, 1) for c in (0, 1)]
+FORMS = [dict(bvh_mode=b, stack_mode=s, csg_stack=c, bvh_radial=r) for b in (0, 1) for s in (0, 1) for c in (0, 1)
+         for r in (0, 1)]
 # (block size, LDS top nodes, workgroups per CU, refill batch)
 SHAPES = [(256, 64, 3, 8), (128, 512, 8, 1), (64, 16, 1, 64)]
 
