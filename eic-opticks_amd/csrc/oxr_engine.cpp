@@ -1144,44 +1144,53 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         if (ms < best_ms) { best_ms = ms; best = t; }
         return OXR_OK;
     };
-    // coordinate search: BVH form and radial culling, launch shape, LDS staging, refill
-    // granularity, the traversal stack form, then the CSG stack depth
-    for (int bm : {0, 1}) {
-        for (int br : {0, 1}) {
-            if (bm == 1 && !e->has_flat) continue;
-            oxr_tuning t = base; t.bvh_mode = bm; t.bvh_radial = br;
-            double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
+    // coordinate search, two passes (the second from the first's best, so a BVH form that
+    // loses at the starting launch shape is tried again at the best one): BVH form and
+    // radial culling, launch shape, LDS staging, refill granularity, the traversal stack
+    // form, then the CSG stack depth. Tunings already measured are not launched again.
+    auto search = [&](const oxr_tuning& start) -> int {
+        for (int bm : {0, 1}) {
+            for (int br : {0, 1}) {
+                if (bm == 1 && !e->has_flat) continue;
+                oxr_tuning t = start; t.bvh_mode = bm; t.bvh_radial = br;
+                double ms; int rc = measure(t, ms); if (rc) return rc;
+            }
         }
-    }
-    const oxr_tuning b0 = best;
-    for (int bs : {64, 128, 256}) {
-        for (int bpc : {1, 2, 3, 4, 6, 8}) {
-            oxr_tuning t = b0; t.block_size = bs; t.blocks_per_cu = bpc;
-            if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, bs), bs) > 64 * 1024) continue;
-            double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
+        const oxr_tuning b0 = best;
+        for (int bs : {64, 128, 256}) {
+            for (int bpc : {1, 2, 3, 4, 6, 8}) {
+                oxr_tuning t = b0; t.block_size = bs; t.blocks_per_cu = bpc;
+                if (oxl::simulate_smem(e->params(e->ev, 0, 0, 0, bs), bs) > 64 * 1024) continue;
+                double ms; int rc = measure(t, ms); if (rc) return rc;
+            }
         }
-    }
-    oxr_tuning b1 = best;
-    for (int nt : {0, 16, 64, 256, 512}) {
-        oxr_tuning t = b1; t.lds_top_nodes = nt;
-        double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
-    }
-    oxr_tuning b2 = best;
-    for (int rb : {1, 8, 16, 32, 48, 64}) {
-        oxr_tuning t = b2; t.refill_batch = rb;
-        double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
-    }
-    oxr_tuning b3 = best;
-    for (int sm : {0, 1}) {
-        oxr_tuning t = b3; t.stack_mode = sm;
-        double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
-    }
-    if (e->registers_csg_ok()) {
-        oxr_tuning b4 = best;
-        for (int cs : {0, 1}) {
-            oxr_tuning t = b4; t.csg_stack = cs;
-            double ms; int rc = measure(t, ms); if (rc) { e->tune = base; return rc; }
+        const oxr_tuning b1 = best;
+        for (int nt : {0, 16, 64, 256, 512}) {
+            oxr_tuning t = b1; t.lds_top_nodes = nt;
+            double ms; int rc = measure(t, ms); if (rc) return rc;
         }
+        const oxr_tuning b2 = best;
+        for (int rb : {1, 8, 16, 32, 48, 64}) {
+            oxr_tuning t = b2; t.refill_batch = rb;
+            double ms; int rc = measure(t, ms); if (rc) return rc;
+        }
+        const oxr_tuning b3 = best;
+        for (int sm : {0, 1}) {
+            oxr_tuning t = b3; t.stack_mode = sm;
+            double ms; int rc = measure(t, ms); if (rc) return rc;
+        }
+        if (e->registers_csg_ok()) {
+            const oxr_tuning b4 = best;
+            for (int cs : {0, 1}) {
+                oxr_tuning t = b4; t.csg_stack = cs;
+                double ms; int rc = measure(t, ms); if (rc) return rc;
+            }
+        }
+        return OXR_OK;
+    };
+    for (int pass = 0; pass < 2; pass++) {
+        int rc = search(pass == 0 ? base : best);
+        if (rc) { e->tune = base; return rc; }
     }
     e->tune = best;
     if (num_results) *num_results = std::min(nres, max_results);   // rows written to results

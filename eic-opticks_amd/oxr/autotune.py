@@ -24,13 +24,15 @@ TABLE = os.path.join(HERE, 'tuning_gfx950.json')
 
 def tune(name, repeat=2, scale=1.0, out=TABLE):
     s = scenes.SCENES[name]()
-    with Engine.from_scene(s) as e:
-        e.set_gensteps(s.gensteps(scale_photons=scale))
-        res, best = e.autotune(repeat=repeat)
     table = {}
     if os.path.exists(out):
         with open(out) as f:
             table = json.load(f)
+    with Engine.from_scene(s) as e:
+        if name in table:
+            e.set_tuning(**table[name])   # the search starts from the recorded best
+        e.set_gensteps(s.gensteps(scale_photons=scale))
+        res, best = e.autotune(repeat=repeat)
     table[name] = best
     table.setdefault('_trials', {})[name] = [dict(block_size=int(r[0]), blocks_per_cu=int(r[1]),
                                                   lds_top_nodes=int(r[2]), refill_batch=int(r[3]),
