@@ -97,9 +97,18 @@ OXR_SIM_EXTERN(true, true)
 #undef OXR_SIM_EXTERN
 // looks up the device address of the no-alloc scratch array for `device` (oxr_noalloc.hip)
 hipError_t noalloc_setup(int device);
+// workgroups of block lanes per CU that can be resident at once for the persistent
+// kernel simulate_persist would launch for P (-1 if the runtime cannot tell)
+int persist_occupancy(const oxd::Params& P, int block, bool fp64, bool list);
 int compact_blocks(long long n);
 hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
                    long long* offs, float4* hit, long long hit_cap);
+// compact() for the serving form: the chunk offsets (nb+1) and the 8 launch counters are
+// also written to host_offs/host_ctr, and hit rows below hcap to hhit, all pinned host
+// memory; rows from hcap on go to dhit (device, dcap rows) at the same row index
+hipError_t compact_host(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
+                        long long* offs, long long* host_offs, const unsigned long long* ctr,
+                        unsigned long long* host_ctr, float4* hhit, long long hcap, float4* dhit, long long dcap);
 // hit selection of the sphotonlite array with the offsets a compact() of the same flags left
 hipError_t scatter_lite(hipStream_t s, const uint8_t* flag, const uint4* lite, long long n, const long long* offs,
                         uint4* hitlite, long long hit_cap);

@@ -341,6 +341,25 @@ hipError_t launch_simulate_plain(hipStream_t s, const oxd::Params& P, dim3 g, di
     return hipGetLastError();
 }
 
+// the persistent kernel instance launch_simulate_persist runs for (debug, tm)
+template <bool FP64, bool LIST>
+const void* persist_kernel(bool debug, int tm) {
+#define OXR_K(D, T) reinterpret_cast<const void*>(&oxd::k_simulate_persist<FP64, D, LIST, T>)
+    if (debug) return (tm & 2) ? OXR_K(true, 2) : OXR_K(true, 0);
+    const bool sh = (tm & 4) && !LIST;
+    switch (tm & 11) {
+        case 0: return sh ? OXR_K(false, 4 * !LIST) : OXR_K(false, 0);
+        case 1: return sh ? OXR_K(false, 1 + 4 * !LIST) : OXR_K(false, 1);
+        case 2: return sh ? OXR_K(false, 2 + 4 * !LIST) : OXR_K(false, 2);
+        case 3: return sh ? OXR_K(false, 3 + 4 * !LIST) : OXR_K(false, 3);
+        case 8: return sh ? OXR_K(false, 8 + 4 * !LIST) : OXR_K(false, 8);
+        case 9: return sh ? OXR_K(false, 9 + 4 * !LIST) : OXR_K(false, 9);
+        case 10: return sh ? OXR_K(false, 10 + 4 * !LIST) : OXR_K(false, 10);
+        default: return sh ? OXR_K(false, 11 + 4 * !LIST) : OXR_K(false, 11);
+    }
+#undef OXR_K
+}
+
 template <bool FP64, bool LIST>
 hipError_t launch_simulate_persist(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool debug, int tm) {
 #define OXR_PERSIST(D, T) hipLaunchKernelGGL((oxd::k_simulate_persist<FP64, D, LIST, T>), g, b, sm, s, P)

@@ -84,7 +84,8 @@ class EngineInfo(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("max_step_photons", C.c_int64), ("hbm_budget_bytes", C.c_int64),
-                ("hit_fraction", C.c_float), ("form", C.c_int32), ("max_active", C.c_int32)]
+                ("hit_fraction", C.c_float), ("form", C.c_int32), ("max_active", C.c_int32),
+                ("headroom", C.c_int32)]
 
 
 class RequestResult(C.Structure):

@@ -245,12 +245,17 @@ class Engine:
     def seed(self):
         return self.get(abi.OXR_GET_SEED)
 
-    def simulate(self, gs, event_index=0, form=abi.OXR_FORM_PLAIN, hit_capacity=None):
-        """One request: gensteps in, hits out (sliced by config.max_photon)."""
+    def simulate(self, gs, event_index=0, form=abi.OXR_FORM_PLAIN, hit_capacity=None, out=None):
+        """One request: gensteps in, hits out (sliced by config.max_photon). out, when
+        given, is a reusable float32 (m, 4, 4) host array the hits are written into (the
+        returned hits are a view of it); it needs room for every photon of the event."""
         gs = np.ascontiguousarray(gs, dtype=np.float32)
         from .gensteps import num_photons
         cap = num_photons(gs) if hit_capacity is None else hit_capacity
-        hits = np.zeros((max(cap, 1), 4, 4), dtype=np.float32)
+        if out is not None:
+            hits = _host_out(out, cap)
+        else:
+            hits = np.zeros((max(cap, 1), 4, 4), dtype=np.float32)
         n = C.c_int64()
         s = abi.LaunchStats()
         _check(lib().oxr_simulate(self.h, abi.ptr(gs), len(gs), event_index, form, abi.ptr(hits), cap,
@@ -289,13 +294,22 @@ class Engine:
         return res[:min(n.value, max_results)], self.tuning()
 
 
+def _host_out(out, cap):
+    if not (isinstance(out, np.ndarray) and out.dtype == np.float32 and out.flags['C_CONTIGUOUS']
+            and out.ndim == 3 and out.shape[1:] == (4, 4)):
+        raise ValueError('out must be a C-contiguous float32 (m, 4, 4) array')
+    if len(out) < max(cap, 1):
+        raise ValueError(f'out holds {len(out)} rows, {cap} needed')
+    return out
+
+
 class Server:
     """Continuous-batching request loop over one resident engine."""
 
     def __init__(self, engine, max_step_photons=4 << 20, hbm_budget_bytes=0, hit_fraction=0.25,
-                 form=abi.OXR_FORM_PERSISTENT, max_active=256):
+                 form=abi.OXR_FORM_PERSISTENT, max_active=256, headroom=False):
         self.engine = engine
-        cfg = abi.ServerConfig(max_step_photons, hbm_budget_bytes, hit_fraction, form, max_active)
+        cfg = abi.ServerConfig(max_step_photons, hbm_budget_bytes, hit_fraction, form, max_active, int(headroom))
         h = C.c_void_p()
         _check(lib().oxr_server_create(engine.h, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -326,8 +340,10 @@ class Server:
         _check(rc)
         return {k: getattr(r, k) for k, _ in r._fields_}
 
-    def take_hits(self, request_id, num_hit):
-        out = np.zeros((max(num_hit, 1), 4, 4), dtype=np.float32)
+    def take_hits(self, request_id, num_hit, out=None):
+        """The request's hits; into out (a reusable float32 (m, 4, 4) host array with
+        m >= num_hit, returned as a view) when given."""
+        out = _host_out(out, num_hit) if out is not None else np.empty((max(num_hit, 1), 4, 4), dtype=np.float32)
         n = C.c_int64()
         _check(lib().oxr_server_take_hits(self.h, request_id, abi.ptr(out), num_hit, C.byref(n)))
         return out[:n.value]

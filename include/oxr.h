@@ -381,6 +381,10 @@ typedef struct oxr_server_config {
     float   hit_fraction;       /* hit buffer reserved per photon (fraction of 64 B)    */
     int32_t form;               /* OXR_FORM_PERSISTENT (default) or OXR_FORM_PLAIN      */
     int32_t max_active;         /* max concurrently admitted requests                   */
+    int32_t headroom;           /* 0 (default): served persistent steps use the tuned    *
+                                 * grid; 1: one workgroup per CU fewer than fit, leaving *
+                                 * room for the other step slot's generate and           *
+                                 * compaction kernels during a step                      */
 } oxr_server_config;
 
 typedef struct oxr_request_result {
@@ -403,13 +407,16 @@ int  oxr_server_submit(oxr_server* s, int64_t request_id, const float* gs, int64
                        int32_t event_index);
 /* Wait for the next completed request (any id). timeout_ms < 0 waits forever. */
 int  oxr_server_wait(oxr_server* s, oxr_request_result* res, int32_t timeout_ms);
-/* Copy the hits of a completed request (sphoton, photon order) and release them; the
- * single device-to-host copy of served hits happens here, on the caller's thread.
+/* Copy the hits of a completed request (sphoton, photon order) into dst (host memory)
+ * and release them. Served hits cross PCIe once, inside the serving pipeline, into
+ * pinned host memory the server owns; this call is a host copy and does no device work.
  * capacity below the count returns OXR_ECAPACITY with *count set, copies nothing and
  * keeps the request for a retry. */
 int  oxr_server_take_hits(oxr_server* s, int64_t request_id, void* dst, int64_t capacity,
                           int64_t* count);
-/* Aggregate: steps run, photons, busy time. out (8) f64. */
+/* Aggregate, out (8) f64: steps run, photons, busy ms (wall time with at least one
+ * step in flight), kernel ms (sum over steps of generate + propagate; steps overlap, so
+ * this can exceed busy), bounce steps, hits, budget bytes in use, budget bytes. */
 int  oxr_server_stats(oxr_server* s, double* out8);
 
 #ifdef __cplusplus

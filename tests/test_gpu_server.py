@@ -143,6 +143,30 @@ def test_admission_budget_serialises_when_tight(engine_mod):
         assert b['t_admit'] >= a['t_done'] - 1e-6
 
 
+def test_hit_rows_beyond_the_host_block(engine_mod):
+    """A step whose hits outnumber the rows its pinned host block was sized for (hit
+    fraction set far below the geometry's): the rows beyond it are fetched from the
+    device buffer, and every event still equals its lone run bit for bit."""
+    s = scenes.instanced_csg()
+    cfg = abi.default_config(**s.config)
+    rng = np.random.default_rng(15)
+    evs = []
+    for k in range(4):
+        ngs = 2
+        u = rng.uniform(size=ngs) ** (1.0 / 3.0) * 17_000.0
+        v = rng.normal(size=(ngs, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        evs.append(G.torch(np.full(ngs, 60_000), pos=v * u[:, None], wavelength=430.0, type=G.T_POINT_ISOTROPIC))
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        ref = [e.simulate(gs, event_index=100 + k)[0] for k, gs in enumerate(evs)]
+        hits, res, st = serve(e, evs, 100, max_step_photons=1 << 20, hit_fraction=1e-4)
+    # the first step's block holds about 4,150 rows; the events hit far more than that
+    assert sum(len(h) for h in ref) > 20000
+    for k in range(len(evs)):
+        assert res[k]['num_hit'] == len(ref[k])
+        assert np.array_equal(bits(hits[k]), bits(ref[k])), k
+
+
 def _check_served(s, cfg, evs, hits, res, alone, o, rng, full_below, sample):
     """Per event: served hits == the event run alone; hits in photon order, each carrying
     its own photon index, all flagged with the hit mask; photon totals conserved. Events
