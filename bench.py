@@ -429,10 +429,10 @@ def main():
         if tune:
             e.set_tuning(**tune)
         evs, counts, arrivals = scenes.serving_stream(np.random.default_rng(42))
-        out = np.empty((int(counts.max()), 4, 4), dtype=np.float32)   # one host hit array, reused
-        out.fill(0.0)                                                   # pages touched before timing
-        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e, evs, arrivals, 4 << 20, out)
-        lat_b, wall_b, ph_b, busy_b = run_sequential(e, evs, arrivals, out)
+        hitbuf = np.empty((int(counts.max()), 4, 4), dtype=np.float32)   # one host hit array, reused
+        hitbuf.fill(0.0)                                                   # pages touched before timing
+        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e, evs, arrivals, 4 << 20, hitbuf)
+        lat_b, wall_b, ph_b, busy_b = run_sequential(e, evs, arrivals, hitbuf)
         e.close()
         out['serving'] = {
             'workload': '256 events, Poisson arrivals 40/s, log-normal photons (median 200k, sigma 1, '
@@ -462,10 +462,10 @@ def main():
         t_one = time.perf_counter()
         e3.launch(abi.OXR_FORM_PERSISTENT)
         one_s = time.perf_counter() - t_one
-        out = np.empty((int(counts.max()), 4, 4), dtype=np.float32)
-        out.fill(0.0)
-        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e3, evs, zero, 8 << 20, out)
-        lat_b, wall_b, ph_b, busy_b = run_sequential(e3, evs, zero, out)
+        hitbuf = np.empty((int(counts.max()), 4, 4), dtype=np.float32)
+        hitbuf.fill(0.0)
+        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e3, evs, zero, 8 << 20, hitbuf)
+        lat_b, wall_b, ph_b, busy_b = run_sequential(e3, evs, zero, hitbuf)
         e3.close()
         out['served_instanced_csg'] = {
             'workload': '128 events submitted at once, log-normal photons (median 200k, sigma 1, clipped 1k-20M), '

@@ -157,6 +157,7 @@ struct Server {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now(), busy_t0;
     int nbusy = 0;
     double last_submit = -1.0;       // server clock of the latest submit (under mu)
+    double prev_submit = -1.0;       // and of the one before it
     double hit_ratio = 0.25;         // hits per photon the next step's host block is sized for
     double stat_steps = 0, stat_photons = 0, stat_busy_ms = 0, stat_kernel_ms = 0, stat_bounce = 0, stat_hits = 0;
     double clock() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
@@ -378,16 +379,17 @@ void Server::fail_all(int rc) {
 }
 
 void Server::loop() {
-    // A step that the pending photons would not fill waits up to COALESCE seconds after
-    // the latest submit (2 x COALESCE at most in all), so a burst of requests is served
-    // in full steps rather than a small one and a full one.
+    // While requests arrive in a burst (the latest two submits within COALESCE seconds),
+    // a step that the pending photons would not fill waits up to COALESCE seconds after
+    // the latest submit (2 x COALESCE at most in all), so the burst is served in full
+    // steps rather than a small one and a full one. A lone request launches at once.
     constexpr double COALESCE = 5e-4;
     bool idle = false;
     double hold = 0.0;               // > 0: hold the launch until this server clock
     double hold_since = -1.0;        // when the current hold began
     for (;;) {
         std::vector<std::unique_ptr<Req>> admitted;
-        double submitted;
+        double submitted, submitted_prev;
         {
             std::unique_lock<std::mutex> lk(mu);
             auto wake = [&] { return stop || !incoming.empty(); };
@@ -402,6 +404,7 @@ void Server::loop() {
             }
             if (stop) break;
             submitted = last_submit;
+            submitted_prev = prev_submit;
             while (!incoming.empty()) {
                 Req& r = *incoming.front();
                 bool room = int(active.size() + admitted.size()) < cfg.max_active;
@@ -438,7 +441,8 @@ void Server::loop() {
                 for (auto& r : active) pending += r->num_photon - r->dispatched;
                 double now = clock();
                 if (hold_since < 0.0) hold_since = now;
-                if (pending < cfg.max_step_photons && now - submitted < COALESCE && now - hold_since < 2 * COALESCE)
+                bool burst = submitted_prev >= 0.0 && submitted - submitted_prev < COALESCE;
+                if (burst && pending < cfg.max_step_photons && now - submitted < COALESCE && now - hold_since < 2 * COALESCE)
                     hold = std::min(submitted + COALESCE, hold_since + 2 * COALESCE);
             }
             if (hold == 0.0) hold_since = -1.0;
@@ -575,6 +579,7 @@ int oxr_server_submit(oxr_server* sh, int64_t id, const float* gs, int64_t num_g
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->finished.count(id)) return fail(OXR_EINVAL, "request id already completed and not taken");
         s->incoming.push_back(std::move(r));
+        s->prev_submit = s->last_submit;
         s->last_submit = s->clock();
     }
     s->cv_work.notify_all();
