@@ -270,6 +270,45 @@ def run_sequential(e, evs, arrivals, out):
     return lat, time.perf_counter() - t0, photons, busy
 
 
+def served_instanced_csg(engine_mod, abi, scenes, s3, cfg3, t3, per_event=True):
+    """128 events submitted at once on the instanced-CSG geometry: served (8M-photon steps)
+    against one persistent launch of all their gensteps (the device rate of this photon
+    mix) and against per-event plain launches one at a time."""
+    e3 = engine_mod.Engine(s3.geometry, s3.physics, cfg3)
+    if t3:
+        e3.set_tuning(**t3)
+    evs, counts, _ = scenes.serving_stream(np.random.default_rng(43), n_events=128, ext=16000.0,
+                                           wavelength=430.0)
+    zero = np.zeros(len(evs))
+    e3.set_gensteps(np.concatenate(evs), event_index=7)
+    e3.launch(abi.OXR_FORM_PERSISTENT)
+    t_one = time.perf_counter()
+    e3.launch(abi.OXR_FORM_PERSISTENT)
+    one_s = time.perf_counter() - t_one
+    hitbuf = np.empty((int(counts.max()), 4, 4), dtype=np.float32)
+    hitbuf.fill(0.0)
+    lat_s, wall_s, ph_s, sst = run_server(engine_mod, e3, evs, zero, 8 << 20, hitbuf)
+    r = {'workload': '128 events submitted at once, log-normal photons (median 200k, sigma 1, clipped 1k-20M), '
+                     'torch gensteps at random interior origins, instanced_csg geometry resident',
+         'events': len(evs), 'photons': int(counts.sum()),
+         'served_photons_per_s': round(ph_s / wall_s, 1),
+         'single_launch_photons_per_s': round(float(counts.sum()) / one_s, 1),
+         'served_vs_single_launch': round((ph_s / wall_s) / (float(counts.sum()) / one_s), 3),
+         'served_p50_ms': round(1e3 * float(np.percentile(lat_s, 50)), 1),
+         'served_p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 1),
+         'served_steps': int(sst['steps']),
+         'served_kernel_ms': round(float(sst['kernel_ms']), 2), 'served_wall_ms': round(1e3 * wall_s, 2),
+         'served_device_busy_frac': round(float(sst['busy_ms']) / (1e3 * wall_s), 3)}
+    if per_event:
+        lat_b, wall_b, ph_b, busy_b = run_sequential(e3, evs, zero, hitbuf)
+        r.update({'per_event_plain_photons_per_s': round(ph_b / wall_b, 1),
+                  'ratio': round((ph_s / wall_s) / (ph_b / wall_b), 3),
+                  'per_event_p50_ms': round(1e3 * float(np.percentile(lat_b, 50)), 1),
+                  'per_event_p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 1)})
+    e3.close()
+    return r
+
+
 WORKLOADS = {
     'instanced_csg': 'instanced_csg: water sphere r=19.5 m in a 120 mm acrylic shell, 20000 instanced 7-node CSG '
                      'solids (zsphere|cylinder minus inner zsphere) at r=19.4 m, 10M torch photons at 430 nm from '
@@ -374,6 +413,8 @@ def main():
     ap.add_argument('--no-precision', action='store_true', help='skip the thin-shell precision and history-chi2 legs')
     ap.add_argument('--workload', choices=sorted(WORKLOADS), default='instanced_csg')
     ap.add_argument('--only', action='store_true', help='time only the --workload launches (for profiling)')
+    ap.add_argument('--served-only', action='store_true',
+                    help='run only the served instanced-CSG stream and its single launch (for profiling)')
     ap.add_argument('--tune', action='append', default=[], metavar='KEY=VALUE',
                     help='with --only: override a field of the committed tuning (e.g. csg_stack=1)')
     args = ap.parse_args()
@@ -386,6 +427,12 @@ def main():
     from oxr import engine as engine_mod
     if args.only:
         only(engine_mod, abi, scenes, args)
+        return
+    if args.served_only:
+        s3 = scenes.instanced_csg()
+        r = served_instanced_csg(engine_mod, abi, scenes, s3, abi.default_config(**s3.config),
+                                 load_tuning('instanced_csg'), per_event=False)
+        print(json.dumps(r), flush=True)
         return
     threads = host_threads()
 
@@ -441,7 +488,7 @@ def main():
             'served': {'p50_ms': round(1e3 * float(np.percentile(lat_s, 50)), 2),
                        'p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 2),
                        'photons_per_s_wall': round(ph_s / wall_s, 1),
-                       'photons_per_s_busy': round(sst['photons'] / (sst['busy_ms'] / 1e3), 1),
+                       'photons_per_s_busy': round(float(sst['photons']) / (float(sst['busy_ms']) / 1e3), 1),
                        'steps': int(sst['steps'])},
             'per_event_plain': {'p50_ms': round(1e3 * float(np.percentile(lat_b, 50)), 2),
                                 'p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 2),
@@ -450,39 +497,7 @@ def main():
         }
         log(f"[bench] serving: {out['serving']}")
         # ------------------------------------------------------------ served vs per-event, instanced CSG
-        e3 = engine_mod.Engine(s3.geometry, s3.physics, cfg3)
-        if t3:
-            e3.set_tuning(**t3)
-        evs, counts, _ = scenes.serving_stream(np.random.default_rng(43), n_events=128, ext=16000.0,
-                                               wavelength=430.0)
-        zero = np.zeros(len(evs))
-        # the device rate of this photon mix: every event's gensteps in one persistent launch
-        e3.set_gensteps(np.concatenate(evs), event_index=7)
-        e3.launch(abi.OXR_FORM_PERSISTENT)
-        t_one = time.perf_counter()
-        e3.launch(abi.OXR_FORM_PERSISTENT)
-        one_s = time.perf_counter() - t_one
-        hitbuf = np.empty((int(counts.max()), 4, 4), dtype=np.float32)
-        hitbuf.fill(0.0)
-        lat_s, wall_s, ph_s, sst = run_server(engine_mod, e3, evs, zero, 8 << 20, hitbuf)
-        lat_b, wall_b, ph_b, busy_b = run_sequential(e3, evs, zero, hitbuf)
-        e3.close()
-        out['served_instanced_csg'] = {
-            'workload': '128 events submitted at once, log-normal photons (median 200k, sigma 1, clipped 1k-20M), '
-                        'torch gensteps at random interior origins, instanced_csg geometry resident',
-            'events': len(evs), 'photons': int(counts.sum()),
-            'served_photons_per_s': round(ph_s / wall_s, 1),
-            'single_launch_photons_per_s': round(float(counts.sum()) / one_s, 1),
-            'served_vs_single_launch': round((ph_s / wall_s) / (float(counts.sum()) / one_s), 3),
-            'per_event_plain_photons_per_s': round(ph_b / wall_b, 1),
-            'ratio': round((ph_s / wall_s) / (ph_b / wall_b), 3),
-            'served_p50_ms': round(1e3 * float(np.percentile(lat_s, 50)), 1),
-            'served_p99_ms': round(1e3 * float(np.percentile(lat_s, 99)), 1),
-            'per_event_p50_ms': round(1e3 * float(np.percentile(lat_b, 50)), 1),
-            'per_event_p99_ms': round(1e3 * float(np.percentile(lat_b, 99)), 1),
-            'served_steps': int(sst['steps']),
-            'served_kernel_ms': round(sst['kernel_ms'], 2), 'served_wall_ms': round(1e3 * wall_s, 2),
-            'served_device_busy_frac': round(sst['busy_ms'] / (1e3 * wall_s), 3)}
+        out['served_instanced_csg'] = served_instanced_csg(engine_mod, abi, scenes, s3, cfg3, t3)
         log(f"[bench] served instanced_csg: {out['served_instanced_csg']}")
 
     if not args.no_precision:

@@ -45,10 +45,11 @@ int genstep_offsets(Engine& e, const float* gs, int64_t num_gs, std::vector<long
 
 // Blocks in power-of-two size classes, kept for reuse until the server ends: device
 // memory for request gensteps, pinned host memory for hits. A request takes the smallest
-// free block of its class or up to four classes above; small device classes are carved
-// from 64-block slabs, one hipMalloc each. New pinned blocks have their pages touched
-// once, so the first host copy out of them does not take the page faults. Used by the
-// serving thread and by take_hits on the caller's thread.
+// free block of its class or above (device blocks at most four classes above); small
+// device classes are carved from 64-block slabs, one hipMalloc each. A new pinned block
+// costs milliseconds (pinning, then touching its pages once so the first host copy out
+// of it takes no page faults), so any free pinned block that fits is used first. Used by
+// the serving thread and by take_hits on the caller's thread.
 template <bool HOST>
 struct Pool {
     std::mutex m;
@@ -60,7 +61,7 @@ struct Pool {
         size_t c = cls(bytes);
         std::lock_guard<std::mutex> lk(m);
         auto it = free_.lower_bound(c);
-        if (it != free_.end() && it->first <= 4 * c) {
+        if (it != free_.end() && (HOST || it->first <= 4 * c)) {
             *p = it->second; *cap = it->first; free_.erase(it); return hipSuccess;
         }
         const int per = (!HOST && c <= (size_t(64) << 10)) ? 64 : 1;
