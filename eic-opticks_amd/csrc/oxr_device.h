@@ -611,8 +611,10 @@ OXD bool leaf_halfspace(Is& is, const NodeV& n, float t_min, f3 o, f3 d) {
     return valid;
 }
 
-// intersect_leaf (CSG/csg_intersect_leaf.h:175-324)
-template <bool FP64>
+// intersect_leaf (CSG/csg_intersect_leaf.h:175-324). SIMPLE compiles only the sphere,
+// zsphere, cylinder and box leaves, for geometries made of nothing else: the other leaves'
+// code and registers then stay out of the kernel.
+template <bool FP64, bool SIMPLE = false>
 OXD bool intersect_leaf(Is& is, const float4* node, int ni, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     NodeV n = load_node(node, ni);
@@ -634,7 +636,15 @@ OXD bool intersect_leaf(Is& is, const float4* node, int ni, const float4* plan, 
                q0.z * rd.x + q1.z * rd.y + q2.z * rd.z + q3.z * 0.f);
     }
     bool valid = false;
-    switch (tc) {
+    if constexpr (SIMPLE) {
+        switch (tc) {
+            case CSG_SPHERE: valid = leaf_sphere<FP64>(is, n, t_min, o, d); break;
+            case CSG_ZSPHERE: valid = leaf_zsphere<FP64>(is, n, t_min, o, d); break;
+            case CSG_CYLINDER: valid = leaf_cylinder<FP64>(is, n, t_min, o, d); break;
+            case CSG_BOX3: valid = leaf_box3(is, n, t_min, o, d); break;
+            default: break;
+        }
+    } else switch (tc) {
         case CSG_SPHERE: valid = leaf_sphere<FP64>(is, n, t_min, o, d); break;
         case CSG_ZSPHERE: valid = leaf_zsphere<FP64>(is, n, t_min, o, d); break;
         case CSG_CYLINDER: valid = leaf_cylinder<FP64>(is, n, t_min, o, d); break;
@@ -800,10 +810,10 @@ OXD bool intersect_node_overlap(Is& is, const float4* node, int ni, int root, co
 }
 
 // intersect_node (csg_intersect_node.h:914-938): list node or leaf
-template <bool FP64, bool LIST>
+template <bool FP64, bool LIST, bool SIMPLE = false>
 OXD bool intersect_node(Is& is, const float4* node, int ni, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
-    if (!LIST) return intersect_leaf<FP64>(is, node, ni, plan, itra, t_min, ro, rd);
+    if (!LIST) return intersect_leaf<FP64, SIMPLE>(is, node, ni, plan, itra, t_min, ro, rd);
     const unsigned tc = __float_as_uint(node[4 * ni + 3].z);
     if (tc == CSG_CONTIGUOUS) { is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f; return intersect_node_contiguous<FP64>(is, node, ni, root, plan, itra, t_min, ro, rd); }
     if (tc == CSG_DISCONTIGUOUS) { is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f; return intersect_node_discontiguous<FP64>(is, node, ni, root, plan, itra, t_min, ro, rd); }
@@ -820,7 +830,7 @@ OXD bool intersect_node(Is& is, const float4* node, int ni, int root, const floa
 // local entries each, selected by if-chains and never indexed, so they live in registers
 // and no private memory is touched. The overflow checks keep the reference's meaning:
 // with these capacities they cannot fire for such trees, as they cannot with its own.
-template <bool FP64, bool LIST, bool SHALLOW = false>
+template <bool FP64, bool LIST, bool SHALLOW = false, bool SIMPLE = false>
 OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     const unsigned numNode = __float_as_uint(node[4 * root].x);
@@ -899,7 +909,7 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
             }
             if (tc >= CSG_NODE) {
                 Is ni_is; ni_is.x = 0.f; ni_is.y = 0.f; ni_is.z = 0.f; ni_is.w = 0.f;
-                intersect_node<FP64, LIST>(ni_is, node, ni, root, plan, itra, tmin, ro, rd);
+                intersect_node<FP64, LIST, SIMPLE>(ni_is, node, ni, root, plan, itra, tmin, ro, rd);
                 ni_is.w = copysignf(ni_is.w, nodeIdx % 2 == 0 ? -1.f : 1.f);
                 if (sp >= NS - 1) { ierr = 1; break; }
                 put(++sp, ni_is);
@@ -954,13 +964,13 @@ OXD bool intersect_tree(Is& out, const float4* node, int root, const float4* pla
 }
 
 // intersect_prim (CSG/csg_intersect_tree.h:684-719)
-template <bool FP64, bool LIST, bool SHALLOW = false>
+template <bool FP64, bool LIST, bool SHALLOW = false, bool SIMPLE = false>
 OXD bool intersect_prim(Is& is, const float4* node, int root, const float4* plan, const float4* itra,
                         float t_min, f3 ro, f3 rd) {
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;   // zeroed isect of CSGOptiX7.cu:894-898
     unsigned tc = __float_as_uint(node[4 * root + 3].z);
-    if (tc >= CSG_LEAF) return intersect_leaf<FP64>(is, node, root, plan, itra, t_min, ro, rd);
-    if (tc < CSG_NODE) return intersect_tree<FP64, LIST, SHALLOW>(is, node, root, plan, itra, t_min, ro, rd);
+    if (tc >= CSG_LEAF) return intersect_leaf<FP64, SIMPLE>(is, node, root, plan, itra, t_min, ro, rd);
+    if (tc < CSG_NODE) return intersect_tree<FP64, LIST, SHALLOW, SIMPLE>(is, node, root, plan, itra, t_min, ro, rd);
     if (LIST) return intersect_node<FP64, LIST>(is, node, root, root, plan, itra, t_min, ro, rd);
     is.x = 0.f; is.y = 0.f; is.z = 0.f; is.w = 0.f;
     return false;

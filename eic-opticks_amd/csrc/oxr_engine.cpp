@@ -109,6 +109,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.stack_cap = stack_cap();
     P.tn_stack = tn ? 1 : 0;
     P.shallow_csg = tune.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
+    P.simple_leaves = simple_leaves && !has_list ? 1 : 0;
     P.radial = tune.bvh_radial == 1 ? 1 : 0;
     P.refine = cfg.propagate_refine ? 1 : 0;
     P.refine_distance = cfg.refine_distance;
@@ -270,14 +271,16 @@ static bool is_op(uint32_t tc) { return tc == 1 || tc == 2 || tc == 3; }
 // arrays: node ranges, tree shape (root subNum = 2^k-1 tree nodes, operators above the
 // leaf level), list nodes (CSG_CONTIGUOUS/DISCONTIGUOUS/OVERLAP: subs inside the prim and
 // all leaves, at most 8 for contiguous), leaf typecodes, transform and plane ranges,
-// boundary indices. Sets e.has_list when any prim holds a list node and e.csg_height to
-// the greatest tree height.
+// boundary indices. Sets e.has_list when any prim holds a list node, e.csg_height to
+// the greatest tree height and e.simple_leaves when every leaf is a sphere, zsphere,
+// cylinder or box.
 static int validate_and_prims(Engine& e, const oxr_geometry* g, const oxr_physics* ph, std::vector<int2>& primtab) {
     const uint32_t* nu = reinterpret_cast<const uint32_t*>(g->node);
     auto node_u = [&](int64_t i, int k) { return nu[16 * i + k]; };
     primtab.resize(size_t(g->num_prim));
     e.has_list = false;
     e.csg_height = 0;
+    e.simple_leaves = true;
     for (int64_t p = 0; p < g->num_prim; p++) {
         const int32_t* pi = reinterpret_cast<const int32_t*>(g->prim + 16 * p);
         const uint32_t* pu = reinterpret_cast<const uint32_t*>(g->prim + 16 * p);
@@ -288,6 +291,7 @@ static int validate_and_prims(Engine& e, const oxr_geometry* g, const oxr_physic
         auto check_leaf = [&](int64_t ni) -> int {
             uint32_t tc = node_u(ni, 14);
             if (!supported_leaf(tc)) return fail(OXR_EINVAL, tag + "unsupported leaf typecode " + std::to_string(tc));
+            if (tc != 101 && tc != 103 && tc != 105 && tc != 110) e.simple_leaves = false;
             uint32_t gt = node_u(ni, 15) & 0x7fffffffu;
             if (int64_t(gt) > g->num_itra) return fail(OXR_EINVAL, tag + "node transform index outside itra");
             if (tc == 112) {

@@ -104,6 +104,7 @@ struct Engine {
     int num_bnd = 0;
     bool has_list = false;   // some prim holds a list node: launches use the LIST kernels
     int csg_height = 0;      // greatest CSG tree height over the prims (0: leaves and lists only)
+    bool simple_leaves = false;   // every leaf is a sphere, zsphere, cylinder or box
     // the register-resident CSG stacks can serve every tree (tuning csg_stack 1)
     bool registers_csg_ok() const { return !has_list && csg_height >= 1 && csg_height <= 2; }
     bool launched = false;

@@ -60,6 +60,7 @@ struct Params {
     int tn_stack;              // stack entries carry their box entry distance (8 B each)
     int shallow_csg;           // every CSG tree has height <= 2 and no prim holds a list node
     int radial;                // nodes also cull by their radial bounds (tuning bvh_radial)
+    int simple_leaves;         // every leaf is a sphere, zsphere, cylinder or box (no list nodes)
     int refine;                // PropagateRefine: trace again from 0.99 of a far hit
     float refine_distance;     // PropagateRefineDistance (mm)
     int refill_threshold;      // persistent form: idle lanes per wave before a refill
@@ -97,6 +98,11 @@ OXR_SIM_EXTERN(true, true)
 #undef OXR_SIM_EXTERN
 // looks up the device address of the no-alloc scratch array for `device` (oxr_noalloc.hip)
 hipError_t noalloc_setup(int device);
+// production kernels of geometries whose leaves are all spheres, zsphere, cylinders or
+// boxes, with single-precision roots and no list nodes (oxr_sim_f32_simple.hip): the
+// traversal mode tm (bits 0-3) plus 16, the simple-leaf bit
+hipError_t launch_simulate_simple(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool persist, int tm);
+const void* persist_kernel_simple(int tm);
 // workgroups of block lanes per CU that can be resident at once for the persistent
 // kernel simulate_persist would launch for P (-1 if the runtime cannot tell)
 int persist_occupancy(const oxd::Params& P, int block, bool fp64, bool list);

@@ -110,7 +110,7 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
 // 0.99 part added back, so a far surface is intersected from close by. A miss reports
 // distance 1, as the miss program writes it (CSGOptiX7.cu:666-670), so h.t is the
 // reference's prd distance in every case. One trace call site serves both passes.
-template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW, bool RADIAL>
+template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW, bool RADIAL, bool SIMPLE = false>
 OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int* stk, int stride,
                        const BNode* top, int ntop, uint32_t& visits, uint32_t& tests, uint32_t& err,
                        uint32_t& csgb, uint32_t& ivis) {
@@ -118,7 +118,7 @@ OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int*
     float t_approx = 0.f;
     bool refined = false, found;
     for (;;) {
-        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW, RADIAL>(P.g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap,
+        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW, RADIAL, SIMPLE>(P.g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap,
                                                                    top, ntop, visits, tests, err, csgb, ivis);
         if (!found) h.t = 1.f;
         if (!P.refine || refined) break;
@@ -138,7 +138,7 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return false;
     float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Hit h;
-    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0, (TM & 8) != 0>(
+    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0, (TM & 8) != 0, (TM & 16) != 0>(
         P, L.p.pos, L.p.mom, tmin, h, stk, stride, top, ntop, visits, tests, err, csgb, ivis);
     if (!found) {
         if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0

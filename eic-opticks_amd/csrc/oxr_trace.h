@@ -100,7 +100,8 @@ OXD void to_object(const InstD& in, f3 pos, f3 mom, f3& o, f3& d) {
 // During traversal only (t, instance, local prim, object-frame normal) of the best hit
 // stay live; its world normal, local position and identity words are formed afterwards
 // from the instance record, with the same arithmetic as at instance entry.
-template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false, bool SHALLOW = false, bool RADIAL = false>
+template <bool FP64, bool COUNT, bool LIST, bool TN = false, bool FLAT = false, bool SHALLOW = false, bool RADIAL = false,
+          bool SIMPLE = false>
 OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                int* stk, int stride, int cap, const BNode* top, int ntop,
                uint32_t& visits, uint32_t& tests, uint32_t& err, uint32_t& csgb, uint32_t& ivis) {
@@ -199,7 +200,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
                 }
             }
             Is is;
-            if (intersect_prim<FP64, LIST, SHALLOW>(is, g.node, nodeOffset, g.plan, g.itra, tmin, lo, ld)) {
+            if (intersect_prim<FP64, LIST, SHALLOW, SIMPLE>(is, g.node, nodeOffset, g.plan, g.itra, tmin, lo, ld)) {
                 float t = is.w;
                 bool closer = t >= tmin && (t < best || (found && t == best && (fi.x < best_inst || (fi.x == best_inst && fi.y < best_item))));
                 if (closer) {
@@ -250,7 +251,7 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
             }
         }
         Is is;
-        if (intersect_prim<FP64, LIST, SHALLOW>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
+        if (intersect_prim<FP64, LIST, SHALLOW, SIMPLE>(is, g.node, nodeOffset, g.plan, g.itra, tmin, o, d)) {
             // a reported distance outside the ray interval is ignored, as by
             // optixReportIntersection (CSGOptiX/CSGOptiX7.cu:912): a convex polyhedron
             // wholly behind the ray reports valid with the zeroed isect
