@@ -26,6 +26,7 @@ void EventBuffers::release_all() {
     rec.release(); prd.release();
     hitflag.release(); hit.release(); counts.release(); offs.release(); counters.release();
     input.release(); pre.release(); lite.release(); hitlite.release(); merged.release(); rstate.release();
+    okey.release(); oval.release(); otmp.release();
 }
 
 static inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
@@ -201,6 +202,18 @@ int run_launch(Engine& e, EventBuffers& b, hipStream_t s, int64_t n, int64_t slo
     OXR_HIP(hipEventRecord(e.eg, s));
     if (form == OXR_FORM_PERSISTENT) {
         int grid = std::max(1, e.tune.blocks_per_cu) * std::max(1, e.num_cu);
+        if (e.tune.photon_order == 1 && gen && !count && P.seed && n > 1 && oxl::production_kernels(P)) {
+            // claim photons sorted by (genstep, direction cell): keys of 16 direction bits
+            // under as many genstep-index bits as the event needs
+            int bits = 16;
+            while (bits < 64 && (int64_t(1) << (bits - 16)) < e.num_gs) bits++;
+            OXR_HIP(b.okey.reserve(2 * size_t(n)));
+            OXR_HIP(b.oval.reserve(2 * size_t(n)));
+            size_t tb = 0;
+            OXR_HIP(oxl::order_photons(s, P, bits, b.okey.p, b.oval.p, nullptr, &tb, nullptr));
+            OXR_HIP(b.otmp.reserve(std::max<size_t>(tb, 1)));
+            OXR_HIP(oxl::order_photons(s, P, bits, b.okey.p, b.oval.p, b.otmp.p, &tb, &P.perm));
+        }
         OXR_HIP(oxl::simulate_persist(s, P, block, grid, e.cfg.fp64_roots != 0, e.has_list));
     } else {
         OXR_HIP(oxl::simulate_plain(s, P, block, e.cfg.fp64_roots != 0, count, e.has_list));
@@ -364,7 +377,7 @@ static bool tuning_ok(const oxr_tuning& t) {
     return t.block_size >= 64 && t.block_size <= 256 && t.block_size % 64 == 0 && t.blocks_per_cu >= 1 &&
            t.blocks_per_cu <= 64 && t.lds_top_nodes >= 0 && t.refill_batch >= 1 && t.refill_batch <= 64 &&
            t.stack_mode >= 0 && t.stack_mode <= 1 && t.bvh_mode >= 0 && t.bvh_mode <= 1 && t.csg_stack >= 0 &&
-           t.csg_stack <= 1 && t.bvh_radial >= 0 && t.bvh_radial <= 1;
+           t.csg_stack <= 1 && t.bvh_radial >= 0 && t.bvh_radial <= 1 && t.photon_order >= 0 && t.photon_order <= 1;
 }
 
 // SEventConfig::HeuristicMaxSlot (sysrap/SEventConfig.cc:1897-1919): with max_photon 0 a
@@ -447,6 +460,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tune.bvh_mode = 0;
     e->tune.csg_stack = 0;
     e->tune.bvh_radial = 0;
+    e->tune.photon_order = 0;
     auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
     if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
@@ -1142,7 +1156,8 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         if (results && nres < max_results) {
             double* r = results + OXR_AUTOTUNE_COLUMNS * nres;
             r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch;
-            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = t.csg_stack; r[7] = t.bvh_radial; r[8] = ms;
+            r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = t.csg_stack; r[7] = t.bvh_radial;
+            r[8] = t.photon_order; r[9] = ms;
         }
         nres++;
         if (ms < best_ms) { best_ms = ms; best = t; }
@@ -1151,7 +1166,8 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
     // coordinate search, two passes (the second from the first's best, so a BVH form that
     // loses at the starting launch shape is tried again at the best one): BVH form and
     // radial culling, launch shape, LDS staging, refill granularity, the traversal stack
-    // form, then the CSG stack depth. Tunings already measured are not launched again.
+    // form, the CSG stack depth, then the photon order. Tunings already measured are not
+    // launched again.
     auto search = [&](const oxr_tuning& start) -> int {
         for (int bm : {0, 1}) {
             for (int br : {0, 1}) {
@@ -1189,6 +1205,11 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
                 oxr_tuning t = b4; t.csg_stack = cs;
                 double ms; int rc = measure(t, ms); if (rc) return rc;
             }
+        }
+        const oxr_tuning b5 = best;
+        for (int po : {0, 1}) {
+            oxr_tuning t = b5; t.photon_order = po;
+            double ms; int rc = measure(t, ms); if (rc) return rc;
         }
         return OXR_OK;
     };

@@ -55,6 +55,9 @@ struct EventBuffers {
     DBuf<uint4> hitlite;       // selected sphotonlite, formed on demand
     DBuf<uint32_t> merged;     // merge output (sphoton or sphotonlite words)
     DBuf<ulonglong2> rstate;   // generate -> propagate hand-over, 16 B per photon
+    DBuf<unsigned long long> okey;   // photon order (tuning photon_order): 2 keys per photon,
+    DBuf<unsigned> oval;             // 2 indices per photon
+    DBuf<char> otmp;                 // and the sort's work space
     void release_all();
 };
 

@@ -61,6 +61,7 @@ struct Params {
     int shallow_csg;           // every CSG tree has height <= 2 and no prim holds a list node
     int radial;                // nodes also cull by their radial bounds (tuning bvh_radial)
     int simple_leaves;         // every leaf is a sphere, zsphere, cylinder or box (no list nodes)
+    const unsigned* perm;      // persistent form: the order photons are claimed in, or null (index order)
     int refine;                // PropagateRefine: trace again from 0.99 of a far hit
     float refine_distance;     // PropagateRefineDistance (mm)
     int refill_threshold;      // persistent form: idle lanes per wave before a refill
@@ -107,6 +108,16 @@ const void* persist_kernel_simple(int tm);
 // kernel simulate_persist would launch for P (-1 if the runtime cannot tell)
 int persist_occupancy(const oxd::Params& P, int block, bool fp64, bool list);
 int compact_blocks(long long n);
+// true when P launches the production kernels (no seqhis, records, rec, prd, photonlite
+// or precooked streams)
+bool production_kernels(const oxd::Params& P);
+// The claim order of a production persistent launch, photons sorted by the key
+// (genstep index << 16) | Morton code of the direction's octahedral cell (256 x 256), read
+// from the generated photons; sorted on key bits [0, key_bits). key and val hold 2 x
+// num_photon entries each, tmp *tmp_bytes; with tmp null only *tmp_bytes is set. *perm
+// receives the sorted photon indices (inside val).
+hipError_t order_photons(hipStream_t s, const oxd::Params& P, int key_bits, unsigned long long* key, unsigned* val,
+                         void* tmp, size_t* tmp_bytes, const unsigned** perm);
 hipError_t compact(hipStream_t s, const uint8_t* flag, const float4* photon, long long n, int* counts,
                    long long* offs, float4* hit, long long hit_cap);
 // compact() for the serving form: the chunk offsets (nb+1) and the 8 launch counters are

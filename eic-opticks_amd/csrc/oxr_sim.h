@@ -251,7 +251,8 @@ __global__ void __launch_bounds__(256, OXR_PLAIN_WAVES) k_simulate(Params P) {
 // ------------------------------------------------------------------ persistent form
 // Lanes hold one photon each and run one bounce step per loop turn. When at least
 // refill_threshold lanes of a wave are idle (or all are), the wave claims that many
-// photon indices with one atomicAdd on counters[1] and the idle lanes generate them.
+// photon indices with one atomicAdd on counters[1] and the idle lanes generate them;
+// with P.perm the k-th claim is photon P.perm[k] (a sorted order), else photon k.
 // Every wave leaves the loop once the counter has passed num_photon and its lanes are idle.
 template <bool FP64, bool DBG, bool LIST, int TM>
 __global__ void __launch_bounds__(256, OXR_PERSIST_WAVES) k_simulate_persist(Params P) {
@@ -279,8 +280,9 @@ __global__ void __launch_bounds__(256, OXR_PERSIST_WAVES) k_simulate_persist(Par
                 unsigned rank = __builtin_amdgcn_mbcnt_hi(uint32_t(idle >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(idle), 0u));
                 unsigned long long my = base + rank;
                 if (my < (unsigned long long)P.num_photon) {
-                    if (start_photon<DBG>(P, L, (long long)my)) active = true;
-                    else P.hitflag[my] = 0;
+                    const long long pi = P.perm ? (long long)P.perm[my] : (long long)my;
+                    if (start_photon<DBG>(P, L, pi)) active = true;
+                    else P.hitflag[pi] = 0;
                 }
             }
         }

@@ -180,6 +180,11 @@ typedef struct oxr_tuning {
                                  range of distances of each child's surfaces from it: exact
                                  for sphere prims, box-based otherwise), which removes the
                                  concentric shells whose boxes all hold the ray origin     */
+    int32_t photon_order;     /* 1: production persistent launches claim photons sorted by
+                                 (genstep, direction cell) after generation, so the lanes of
+                                 a wave trace rays from one origin in neighbouring
+                                 directions (results unchanged: every photon keeps its
+                                 index and random stream); 0: index order                 */
 } oxr_tuning;
 
 /* ---- outputs ---------------------------------------------------------- */
@@ -361,11 +366,11 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
  * Times the persistent form on the current event over a grid of oxr_tuning values
  * (repeat times each) and applies the fastest. results (max_results,OXR_AUTOTUNE_COLUMNS)
  * f64 = block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode,
- * csg_stack, bvh_radial, kernel_ms,
+ * csg_stack, bvh_radial, photon_order, kernel_ms,
  * one row per distinct effective tuning launched (see oxr_engine_set_tuning);
  * *num_results = rows written (at most max_results). On an error the tuning in force
  * before the call is restored. No reference counterpart. */
-#define OXR_AUTOTUNE_COLUMNS 9
+#define OXR_AUTOTUNE_COLUMNS 10
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 

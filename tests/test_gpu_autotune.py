@@ -10,7 +10,8 @@ from oxr import abi, scenes
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ['block_size', 'blocks_per_cu', 'lds_top_nodes', 'refill_batch', 'stack_mode', 'bvh_mode', 'csg_stack', 'bvh_radial']
+KEYS = ['block_size', 'blocks_per_cu', 'lds_top_nodes', 'refill_batch', 'stack_mode', 'bvh_mode', 'csg_stack', 'bvh_radial',
+        'photon_order']
 
 
 def test_autotune_picks_the_fastest_row_and_keeps_parity(engine_mod, oracle_lib):
@@ -21,13 +22,14 @@ def test_autotune_picks_the_fastest_row_and_keeps_parity(engine_mod, oracle_lib)
         e.set_gensteps(gs)
         print('autotune, instanced CSG, 30k photons', flush=True)
         rows, best = e.autotune(repeat=1)
-        assert len(rows) >= 8 and np.all(rows[:, 8] > 0)
-        assert len({tuple(r[:8]) for r in rows}) == len(rows)
-        k = int(np.argmin(rows[:, 8]))
-        assert best == dict(zip(KEYS, (int(v) for v in rows[k, :8])))
+        assert len(rows) >= 8 and np.all(rows[:, 9] > 0)
+        assert len({tuple(r[:9]) for r in rows}) == len(rows)
+        k = int(np.argmin(rows[:, 9]))
+        assert best == dict(zip(KEYS, (int(v) for v in rows[k, :9])))
         assert e.tuning() == best
-        # the CSG stack depth and the radial culling were searched dimensions
+        # the CSG stack depth, the radial culling and the photon order were searched dimensions
         assert set(rows[:, 6].astype(int)) == {0, 1} and set(rows[:, 7].astype(int)) == {0, 1}
+        assert set(rows[:, 8].astype(int)) == {0, 1}
         e.launch(abi.OXR_FORM_PERSISTENT)
         got = e.photons()
         few, _ = e.autotune(repeat=1, max_results=3)
