@@ -757,6 +757,8 @@ int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
     std::strncpy(info->arch, e->arch.c_str(), sizeof(info->arch) - 1);
     info->csg_height = e->csg_height;
     info->csg_in_registers = e->tune.csg_stack == 1 && e->registers_csg_ok() ? 1 : 0;
+    info->simple_leaves = e->simple_leaves && !e->has_list ? 1 : 0;
+    info->reserved = 0;
     return OXR_OK;
 }
 

@@ -80,7 +80,8 @@ class EngineInfo(C.Structure):
                 ("node_bytes", C.c_int32), ("num_cu", C.c_int32), ("device_bytes", C.c_int64),
                 ("free_bytes", C.c_int64), ("total_bytes", C.c_int64),
                 ("device_name", C.c_char * 64), ("arch", C.c_char * 32),
-                ("csg_height", C.c_int32), ("csg_in_registers", C.c_int32)]
+                ("csg_height", C.c_int32), ("csg_in_registers", C.c_int32), ("simple_leaves", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class ServerConfig(C.Structure):

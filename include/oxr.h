@@ -217,6 +217,10 @@ typedef struct oxr_engine_info {
     int32_t csg_in_registers; /* 1: production launches keep the CSG evaluation stacks in
                                  registers (tuning csg_stack 1 in force: every tree of height
                                  1 or 2, no list nodes)                                       */
+    int32_t simple_leaves;    /* 1: every leaf is a sphere, zsphere, cylinder or box and no
+                                 prim holds a list node, so production launches with single-
+                                 precision roots run kernels compiled with those leaves only */
+    int32_t reserved;
 } oxr_engine_info;
 
 typedef struct oxr_engine oxr_engine;

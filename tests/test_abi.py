@@ -31,7 +31,7 @@ def test_struct_sizes_match_header(engine_mod):
     assert C.sizeof(abi.Tuning) == 36
     assert C.sizeof(abi.RequestResult) == 64
     assert C.sizeof(abi.ServerConfig) == 32 and abi.ServerConfig.headroom.offset == 28
-    assert C.sizeof(abi.EngineInfo) == 184
+    assert C.sizeof(abi.EngineInfo) == 192
 
 
 def test_config_defaults_are_reference_defaults(engine_mod):

@@ -451,6 +451,41 @@ def test_csg_stacks_in_registers_for_shallow_trees(engine_mod):
     assert (i['csg_height'], i['csg_in_registers']) == (3, 0)
 
 
+def test_simple_leaf_kernels_chosen_by_the_leaves(engine_mod, oracle_lib):
+    """A geometry whose leaves are all spheres, zspheres, cylinders or boxes runs the
+    kernels compiled with those leaves only (every measured event does); one cone leaf
+    anywhere selects the general kernels. Photons of both equal the oracle's."""
+    for name in ('instanced_csg', 'thin_shells', 'csi_calorimeter', 'nested_boxes'):
+        s = scene(name)
+        with engine_mod.Engine(s.geometry, s.physics, abi.default_config(**s.config)) as e:
+            assert e.info()['simple_leaves'] == 1, name
+    for with_cone in (False, True):
+        f = Foundry()
+        f.add_material('Air', 1.0)
+        f.add_material('Glass', 1.5)
+        b0 = f.boundary('Air', None, None, 'Air')
+        b1 = f.boundary('Air', None, None, 'Glass')
+        f.begin_solid('r0')
+        f.add_prim(box3(1000.0), b0)
+        f.add_prim(sphere(60.0) | cylinder(30.0, -90.0, 90.0), b1)
+        if with_cone:
+            f.add_prim(cone(40.0, -20.0, 10.0, 20.0).moved(translate(0.0, 0.0, 200.0)), b1)
+        f.end_solid()
+        f.add_instance(np.eye(4), 0)
+        g, p = f.arrays(), f.physics()
+        gs = G.torch([4000], pos=(0.0, 0.0, -300.0), wavelength=440.0, type=G.T_POINT_ISOTROPIC)
+        cfg = abi.default_config()
+        with engine_mod.Engine(g, p, cfg) as e:
+            assert e.info()['simple_leaves'] == (0 if with_cone else 1)
+            e.set_gensteps(gs)
+            e.launch(abi.OXR_FORM_PERSISTENT)
+            got = e.photons()
+        o = oracle_lib.Oracle(g, p, cfg)
+        o.set_cull(True)
+        want = o.simulate(gs, seq=False)['photon']
+        assert np.array_equal(bits(got), bits(want)), with_cone
+
+
 def test_max_photon_from_device_memory(engine_mod):
     """max_photon 0 (the default, as the reference's MaxSlot 0) sizes a launch slice from
     the device memory: 87% of it over the bytes per photon, rounded down to a million
