@@ -73,6 +73,7 @@ oxr_tuning Engine::effective(const oxr_tuning& t) const {
     r.stack_mode = tn ? 1 : 0;
     r.lds_top_nodes = tn ? int(std::min<int64_t>(std::max(0, t.lds_top_nodes), top_nodes(t))) : ntop_for(t, t.block_size);
     r.csg_stack = t.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
+    r.photon_order = t.photon_order == 1 && t.generate_inplace != 1 ? 1 : 0;   // nothing to sort in place
     return r;
 }
 
@@ -111,6 +112,7 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.tn_stack = tn ? 1 : 0;
     P.shallow_csg = tune.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
     P.simple_leaves = simple_leaves && !has_list ? 1 : 0;
+    P.gen_inplace = tune.generate_inplace == 1 ? 1 : 0;
     P.radial = tune.bvh_radial == 1 ? 1 : 0;
     P.refine = cfg.propagate_refine ? 1 : 0;
     P.refine_distance = cfg.refine_distance;
@@ -377,7 +379,8 @@ static bool tuning_ok(const oxr_tuning& t) {
     return t.block_size >= 64 && t.block_size <= 256 && t.block_size % 64 == 0 && t.blocks_per_cu >= 1 &&
            t.blocks_per_cu <= 64 && t.lds_top_nodes >= 0 && t.refill_batch >= 1 && t.refill_batch <= 64 &&
            t.stack_mode >= 0 && t.stack_mode <= 1 && t.bvh_mode >= 0 && t.bvh_mode <= 1 && t.csg_stack >= 0 &&
-           t.csg_stack <= 1 && t.bvh_radial >= 0 && t.bvh_radial <= 1 && t.photon_order >= 0 && t.photon_order <= 1;
+           t.csg_stack <= 1 && t.bvh_radial >= 0 && t.bvh_radial <= 1 && t.photon_order >= 0 && t.photon_order <= 1 &&
+           t.generate_inplace >= 0 && t.generate_inplace <= 1;
 }
 
 // SEventConfig::HeuristicMaxSlot (sysrap/SEventConfig.cc:1897-1919): with max_photon 0 a
@@ -461,6 +464,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     e->tune.csg_stack = 0;
     e->tune.bvh_radial = 0;
     e->tune.photon_order = 0;
+    e->tune.generate_inplace = 0;
     auto bail = [&](int rc) { oxr_engine_destroy(reinterpret_cast<oxr_engine*>(e)); return rc; };
     if ((he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking)) != hipSuccess) return bail(hip_fail(he, "hipStreamCreate"));
     if ((he = hipEventCreate(&e->e0)) != hipSuccess) return bail(hip_fail(he, "hipEventCreate"));
@@ -1159,7 +1163,7 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
             double* r = results + OXR_AUTOTUNE_COLUMNS * nres;
             r[0] = t.block_size; r[1] = t.blocks_per_cu; r[2] = t.lds_top_nodes; r[3] = t.refill_batch;
             r[4] = t.stack_mode; r[5] = t.bvh_mode; r[6] = t.csg_stack; r[7] = t.bvh_radial;
-            r[8] = t.photon_order; r[9] = ms;
+            r[8] = t.photon_order; r[9] = t.generate_inplace; r[10] = ms;
         }
         nres++;
         if (ms < best_ms) { best_ms = ms; best = t; }
@@ -1168,8 +1172,8 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
     // coordinate search, two passes (the second from the first's best, so a BVH form that
     // loses at the starting launch shape is tried again at the best one): BVH form and
     // radial culling, launch shape, LDS staging, refill granularity, the traversal stack
-    // form, the CSG stack depth, then the photon order. Tunings already measured are not
-    // launched again.
+    // form, the CSG stack depth, the photon order, then where photons are generated.
+    // Tunings already measured are not launched again.
     auto search = [&](const oxr_tuning& start) -> int {
         for (int bm : {0, 1}) {
             for (int br : {0, 1}) {
@@ -1211,6 +1215,11 @@ int oxr_autotune(oxr_engine* eh, int32_t repeat, double* results, int32_t max_re
         const oxr_tuning b5 = best;
         for (int po : {0, 1}) {
             oxr_tuning t = b5; t.photon_order = po;
+            double ms; int rc = measure(t, ms); if (rc) return rc;
+        }
+        const oxr_tuning b6 = best;
+        for (int gi : {0, 1}) {
+            oxr_tuning t = b6; t.generate_inplace = gi;
             double ms; int rc = measure(t, ms); if (rc) return rc;
         }
         return OXR_OK;

@@ -62,6 +62,7 @@ struct Params {
     int radial;                // nodes also cull by their radial bounds (tuning bvh_radial)
     int simple_leaves;         // every leaf is a sphere, zsphere, cylinder or box (no list nodes)
     const unsigned* perm;      // persistent form: the order photons are claimed in, or null (index order)
+    int gen_inplace;           // production kernels generate each photon where they claim it
     int refine;                // PropagateRefine: trace again from 0.99 of a far hit
     float refine_distance;     // PropagateRefineDistance (mm)
     int refill_threshold;      // persistent form: idle lanes per wave before a refill

@@ -180,6 +180,7 @@ OXD bool resume_photon(const Params& P, Lane& L, long long idx) {
 template <bool DBG>
 OXD bool start_photon(const Params& P, Lane& L, long long idx) {
     if (DBG) return begin_photon<true>(P, L, idx);
+    if (P.gen_inplace) return begin_photon<false>(P, L, idx);
     return resume_photon(P, L, idx);
 }
 

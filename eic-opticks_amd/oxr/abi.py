@@ -14,7 +14,7 @@ OXR_RNG_ALIGNED = 1
 OXR_NOALLOC_MAX_PHOTON = 1 << 19
 OXR_NOALLOC_MAX_GENSTEP = 1 << 14
 OXR_NOALLOC_SCRATCH_BYTES = 12716288
-OXR_AUTOTUNE_COLUMNS = 10
+OXR_AUTOTUNE_COLUMNS = 11
 (OXR_GET_PHOTON, OXR_GET_HIT, OXR_GET_SEQ, OXR_GET_RECORD, OXR_GET_SEED, OXR_GET_GENSTEP,
  OXR_GET_PHOTONLITE, OXR_GET_HITLITE, OXR_GET_REC, OXR_GET_PRD) = range(10)
 
@@ -64,7 +64,7 @@ class Tuning(C.Structure):
     _fields_ = [("block_size", C.c_int32), ("blocks_per_cu", C.c_int32),
                 ("lds_top_nodes", C.c_int32), ("refill_batch", C.c_int32), ("stack_mode", C.c_int32),
                 ("bvh_mode", C.c_int32), ("csg_stack", C.c_int32), ("bvh_radial", C.c_int32),
-                ("photon_order", C.c_int32)]
+                ("photon_order", C.c_int32), ("generate_inplace", C.c_int32)]
 
 
 class LaunchStats(C.Structure):

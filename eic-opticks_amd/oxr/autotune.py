@@ -38,7 +38,8 @@ def tune(name, repeat=2, scale=1.0, out=TABLE):
                                                   lds_top_nodes=int(r[2]), refill_batch=int(r[3]),
                                                   stack_mode=int(r[4]), bvh_mode=int(r[5]),
                                                   csg_stack=int(r[6]), bvh_radial=int(r[7]),
-                                                  photon_order=int(r[8]), kernel_ms=round(float(r[9]), 4))
+                                                  photon_order=int(r[8]), generate_inplace=int(r[9]),
+                                                  kernel_ms=round(float(r[10]), 4))
                                              for r in res]
     with open(out, 'w') as f:
         json.dump(table, f, indent=1, sort_keys=True)

@@ -185,6 +185,10 @@ typedef struct oxr_tuning {
                                  a wave trace rays from one origin in neighbouring
                                  directions (results unchanged: every photon keeps its
                                  index and random stream); 0: index order                 */
+    int32_t generate_inplace; /* 1: production launches generate each photon where the
+                                 bounce loop claims it, with no generation pass before (so
+                                 photon_order has nothing to sort and runs as 0); 0: a
+                                 generation pass first, whose photons the loop resumes    */
 } oxr_tuning;
 
 /* ---- outputs ---------------------------------------------------------- */
@@ -370,11 +374,11 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
  * Times the persistent form on the current event over a grid of oxr_tuning values
  * (repeat times each) and applies the fastest. results (max_results,OXR_AUTOTUNE_COLUMNS)
  * f64 = block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode,
- * csg_stack, bvh_radial, photon_order, kernel_ms,
+ * csg_stack, bvh_radial, photon_order, generate_inplace, kernel_ms,
  * one row per distinct effective tuning launched (see oxr_engine_set_tuning);
  * *num_results = rows written (at most max_results). On an error the tuning in force
  * before the call is restored. No reference counterpart. */
-#define OXR_AUTOTUNE_COLUMNS 10
+#define OXR_AUTOTUNE_COLUMNS 11
 int oxr_autotune(oxr_engine* e, int32_t repeat, double* results, int32_t max_results,
                  int32_t* num_results);
 

@@ -28,7 +28,7 @@ def test_struct_sizes_match_header(engine_mod):
     assert C.sizeof(abi.Geometry) == 96
     assert C.sizeof(abi.Config) == 120
     assert C.sizeof(abi.LaunchStats) == 88
-    assert C.sizeof(abi.Tuning) == 36
+    assert C.sizeof(abi.Tuning) == 40
     assert C.sizeof(abi.RequestResult) == 64
     assert C.sizeof(abi.ServerConfig) == 32 and abi.ServerConfig.headroom.offset == 28
     assert C.sizeof(abi.EngineInfo) == 192

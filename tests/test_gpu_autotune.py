@@ -11,7 +11,7 @@ from oxr import abi, scenes
 pytestmark = pytest.mark.gpu
 
 KEYS = ['block_size', 'blocks_per_cu', 'lds_top_nodes', 'refill_batch', 'stack_mode', 'bvh_mode', 'csg_stack', 'bvh_radial',
-        'photon_order']
+        'photon_order', 'generate_inplace']
 
 
 def test_autotune_picks_the_fastest_row_and_keeps_parity(engine_mod, oracle_lib):
@@ -22,14 +22,15 @@ def test_autotune_picks_the_fastest_row_and_keeps_parity(engine_mod, oracle_lib)
         e.set_gensteps(gs)
         print('autotune, instanced CSG, 30k photons', flush=True)
         rows, best = e.autotune(repeat=1)
-        assert len(rows) >= 8 and np.all(rows[:, 9] > 0)
-        assert len({tuple(r[:9]) for r in rows}) == len(rows)
-        k = int(np.argmin(rows[:, 9]))
-        assert best == dict(zip(KEYS, (int(v) for v in rows[k, :9])))
+        assert len(rows) >= 8 and np.all(rows[:, 10] > 0)
+        assert len({tuple(r[:10]) for r in rows}) == len(rows)
+        k = int(np.argmin(rows[:, 10]))
+        assert best == dict(zip(KEYS, (int(v) for v in rows[k, :10])))
         assert e.tuning() == best
-        # the CSG stack depth, the radial culling and the photon order were searched dimensions
+        # the CSG stack depth, the radial culling, the photon order and where photons are
+        # generated were searched dimensions
         assert set(rows[:, 6].astype(int)) == {0, 1} and set(rows[:, 7].astype(int)) == {0, 1}
-        assert set(rows[:, 8].astype(int)) == {0, 1}
+        assert set(rows[:, 8].astype(int)) == {0, 1} and set(rows[:, 9].astype(int)) == {0, 1}
         e.launch(abi.OXR_FORM_PERSISTENT)
         got = e.photons()
         few, _ = e.autotune(repeat=1, max_results=3)

@@ -1,8 +1,8 @@
 """Every traversal form the tuning can select (two-level or flattened BVH, plain or
 distance-carrying stack, CSG stacks in registers or private memory, radial culling) and
 launch shape (workgroup size, LDS staging, workgroups per CU, refill batch, photons claimed
-in index or sorted order) gives the same photons, bit for bit, and the same photons as the
-oracle."""
+in index or sorted order, generated in a pass of their own or where they are claimed) gives
+the same photons, bit for bit, and the same photons as the oracle."""
 import numpy as np
 import pytest
 
@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 
 FORMS = [dict(bvh_mode=b, stack_mode=s, csg_stack=c, bvh_radial=r) for b in (0, 1) for s in (0, 1) for c in (0, 1)
          for r in (0, 1)]
-# (block size, LDS top nodes, workgroups per CU, refill batch, photon order)
-SHAPES = [(256, 64, 3, 8, 0), (128, 512, 8, 1, 1), (64, 16, 1, 64, 1)]
+# (block size, LDS top nodes, workgroups per CU, refill batch, photon order, generation in place)
+SHAPES = [(256, 64, 3, 8, 0, 0), (128, 512, 8, 1, 1, 0), (64, 16, 1, 64, 0, 1)]
 
 
 def cases():
@@ -40,11 +40,11 @@ def test_forms_agree(engine_mod, oracle_lib, case):
     with engine_mod.Engine(geometry, physics, cfg) as e:
         e.set_gensteps(gs)
         for form in FORMS:
-            for block, top, bpc, refill, order in SHAPES:
+            for block, top, bpc, refill, order, inplace in SHAPES:
                 e.set_tuning(block_size=block, lds_top_nodes=top, blocks_per_cu=bpc, refill_batch=refill,
-                             photon_order=order, **form)
+                             photon_order=order, generate_inplace=inplace, **form)
                 for kind in (abi.OXR_FORM_PERSISTENT, abi.OXR_FORM_PLAIN):
-                    print(name, form, block, top, order, kind, flush=True)
+                    print(name, form, block, top, order, inplace, kind, flush=True)
                     e.launch(kind)
                     got = e.photons()
                     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (name, form, block, kind)
