@@ -333,7 +333,8 @@ def roofline_block(e, name, n, n_gs, n_hit, kms, generate_ms):
     traffic = prof.get('traffic')
     return dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel='k_generate + k_simulate_persist (one simulate launch)',
+                kernel=('k_simulate_persist generating in place (one simulate launch)' if e.tuning().get('generate_inplace')
+                        else 'k_generate + k_simulate_persist (one simulate launch)'),
                 generate_ms=round(generate_ms, 4), propagate_ms=round(kms - generate_ms, 4),
                 algorithmic_bytes_per_launch=int(ab),
                 node_visits=int(cnt['node_visits']), prim_tests=int(cnt['prim_tests']),
