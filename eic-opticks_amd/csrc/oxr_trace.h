@@ -19,6 +19,24 @@
 namespace oxd {
 
 struct BNode { float4 a, b, c; int4 ch; };
+// A staged node is read through an LDS-typed pointer, so it loads with ds_read; a load
+// through one pointer that may address either LDS or global memory compiles to a flat load,
+// which takes the vector-memory path for both.
+typedef float LdsF4 __attribute__((ext_vector_type(4)));
+typedef int LdsI4 __attribute__((ext_vector_type(4)));
+struct LdsNode { LdsF4 a, b, c; LdsI4 ch; };
+typedef __attribute__((address_space(3))) const LdsNode LdsNodeL;
+OXD BNode lds_node(const BNode* top, int i) {
+    LdsNodeL* p = (LdsNodeL*)top + i;
+    const LdsF4 a = p->a, b = p->b, c = p->c;
+    const LdsI4 ch = p->ch;
+    BNode n;
+    n.a = make_float4(a.x, a.y, a.z, a.w);
+    n.b = make_float4(b.x, b.y, b.z, b.w);
+    n.c = make_float4(c.x, c.y, c.z, c.w);
+    n.ch = make_int4(ch.x, ch.y, ch.z, ch.w);
+    return n;
+}
 struct InstD { float4 c0, c1, c2; int4 info; };   // info: prim offset, ii_id, flags (bit 0 identity,
                                                   // bits 8-15 visibility mask), blas_root
 // an instance takes part in a trace when its visibility mask shares a bit with the ray's
@@ -129,8 +147,8 @@ OXD bool trace(const Geom& g, f3 pos, f3 mom, float tmin, float tmax, Hit& hit,
     for (;;) {
         if (cur >= 0) {
             BNode nd;
-            if (at_top) nd = cur < ntop ? top[cur] : g.tlas[cur];
-            else nd = g.blas[cur];
+            if (at_top && cur < ntop) nd = lds_node(top, cur);
+            else nd = at_top ? g.tlas[cur] : g.blas[cur];
             if (COUNT) visits++;
             float tn0, tn1;
             bool h0 = box_hit(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, noi, inv, best, tn0);
