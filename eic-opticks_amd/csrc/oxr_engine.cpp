@@ -66,6 +66,19 @@ bool Engine::tn_stack_for(const oxr_tuning& t, int block) const {
     return need <= share;
 }
 
+// The simple-leaf kernels read the node and itra arrays from a copy in LDS when the arrays
+// take at most 16 KB and the copy leaves as many blocks of `block` lanes resident per CU
+// as the launch has without it (smem bytes): 160 KB of LDS per CU, and at most 16 waves
+// per CU at the kernels' 4 waves per SIMD. A copy that cost a resident block would cost
+// more in latency hiding than its LDS reads save.
+bool Engine::csg_copy_for(size_t smem, int block) const {
+    const size_t bytes = size_t(num_node + num_itra) * 64;
+    if (bytes > 16 * 1024 || smem + bytes > 64 * 1024 || block < 64) return false;
+    const size_t waves = size_t(16 / (block / 64));
+    auto resident = [&](size_t sm) { return std::min(waves, size_t(160 * 1024) / std::max<size_t>(sm, 1)); };
+    return resident(smem + bytes) >= resident(smem);
+}
+
 oxr_tuning Engine::effective(const oxr_tuning& t) const {
     oxr_tuning r = t;
     r.bvh_mode = use_flat(t) ? 1 : 0;
@@ -112,6 +125,11 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.tn_stack = tn ? 1 : 0;
     P.shallow_csg = tune.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
     P.simple_leaves = simple_leaves && !has_list ? 1 : 0;
+    P.csg_node4 = P.csg_itra4 = 0;
+    if (P.simple_leaves && csg_copy_for(oxl::simulate_smem(P, block), block)) {
+        P.csg_node4 = num_node * 4;
+        P.csg_itra4 = num_itra * 4;
+    }
     P.gen_inplace = tune.generate_inplace == 1 ? 1 : 0;
     P.radial = tune.bvh_radial == 1 ? 1 : 0;
     P.refine = cfg.propagate_refine ? 1 : 0;
@@ -476,6 +494,7 @@ int oxr_engine_create(const oxr_geometry* g, const oxr_physics* ph, const oxr_co
     int rc = validate_and_prims(*e, g, ph, primtab);
     if (rc) return bail(rc);
     e->num_prim = int(g->num_prim); e->num_node = int(g->num_node); e->num_solid = int(g->num_solid);
+    e->num_itra = int(g->num_itra);
     e->num_inst = int(g->num_inst);
     e->num_bnd = ph->num_bnd;
 
@@ -762,7 +781,8 @@ int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
     info->csg_height = e->csg_height;
     info->csg_in_registers = e->tune.csg_stack == 1 && e->registers_csg_ok() ? 1 : 0;
     info->simple_leaves = e->simple_leaves && !e->has_list ? 1 : 0;
-    info->reserved = 0;
+    info->csg_in_lds = info->simple_leaves && !e->cfg.fp64_roots &&
+                       e->params(e->ev, 0, 0, 0, e->tune.block_size).csg_node4 > 0 ? 1 : 0;
     return OXR_OK;
 }
 

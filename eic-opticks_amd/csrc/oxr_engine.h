@@ -88,7 +88,7 @@ struct Engine {
     DBuf<uint4> optical;
     DBuf<float> icdf;
     int tlas_root = 0;
-    int num_inst = 0, num_solid = 0, num_prim = 0, num_node = 0;
+    int num_inst = 0, num_solid = 0, num_prim = 0, num_node = 0, num_itra = 0;
     int64_t tlas_nodes = 0, blas_nodes = 0;
     int stack_need = 0;
     std::vector<int> solid_prim_offset, solid_num_prim, solid_blas_root;
@@ -124,6 +124,7 @@ struct Engine {
     int stack_cap(const oxr_tuning& t) const;
     int stack_cap() const { return stack_cap(tune); }
     int ntop_for(const oxr_tuning& t, int block) const;
+    bool csg_copy_for(size_t smem, int block) const;
     bool tn_stack_for(const oxr_tuning& t, int block) const;
     // t as the launches of the persistent form run it: stack_mode 1 only where the
     // distance-carrying stack fits (tn_stack_for), lds_top_nodes as many as are staged

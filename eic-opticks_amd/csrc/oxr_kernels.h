@@ -61,6 +61,8 @@ struct Params {
     int shallow_csg;           // every CSG tree has height <= 2 and no prim holds a list node
     int radial;                // nodes also cull by their radial bounds (tuning bvh_radial)
     int simple_leaves;         // every leaf is a sphere, zsphere, cylinder or box (no list nodes)
+    int csg_node4, csg_itra4;  // > 0: the simple-leaf kernels copy the node and itra arrays (this many
+                               // float4) into LDS after the traversal stacks and read them there
     const unsigned* perm;      // persistent form: the order photons are claimed in, or null (index order)
     int gen_inplace;           // production kernels generate each photon where they claim it
     int refine;                // PropagateRefine: trace again from 0.99 of a far hit
@@ -100,11 +102,20 @@ OXR_SIM_EXTERN(true, true)
 #undef OXR_SIM_EXTERN
 // looks up the device address of the no-alloc scratch array for `device` (oxr_noalloc.hip)
 hipError_t noalloc_setup(int device);
-// production kernels of geometries whose leaves are all spheres, zsphere, cylinders or
-// boxes, with single-precision roots and no list nodes (oxr_sim_f32_simple.hip): the
-// traversal mode tm (bits 0-3) plus 16, the simple-leaf bit
+// production kernels of geometries whose leaves are all spheres, zspheres, cylinders or
+// boxes, with single-precision roots and no list nodes: the traversal mode tm (bits 0-3)
+// plus BASE, 16 (the simple-leaf bit; oxr_sim_f32_simple.hip) or 48 (also bit 5: node and
+// itra arrays read from their LDS copy, P.csg_node4 > 0; oxr_sim_f32_simple_lds.hip)
+template <int BASE>
 hipError_t launch_simulate_simple(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool persist, int tm);
+template <int BASE>
 const void* persist_kernel_simple(int tm);
+#define OXR_SIMPLE_EXTERN(B)                                                                                                 \
+    extern template hipError_t launch_simulate_simple<B>(hipStream_t, const oxd::Params&, dim3, dim3, size_t, bool, int); \
+    extern template const void* persist_kernel_simple<B>(int);
+OXR_SIMPLE_EXTERN(16)
+OXR_SIMPLE_EXTERN(48)
+#undef OXR_SIMPLE_EXTERN
 // workgroups of block lanes per CU that can be resident at once for the persistent
 // kernel simulate_persist would launch for P (-1 if the runtime cannot tell)
 int persist_occupancy(const oxd::Params& P, int block, bool fp64, bool list);

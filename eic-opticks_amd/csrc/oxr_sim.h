@@ -60,6 +60,14 @@ OXD void point(const Params& P, Lane& L) {
     }
 }
 
+// the LDS copy of the node and itra arrays (TM bit 5): after the P.ntop staged top nodes
+// and the traversal stacks (blockDim.x lanes x P.stack_cap entries of 4 B, 8 B with
+// distances), the layout simulate_smem sizes
+OXD const float4* csg_copy(const Params& P) {
+    extern __shared__ int4 smem4[];
+    return reinterpret_cast<const float4*>(smem4) + 4 * P.ntop + (blockDim.x * unsigned(P.stack_cap) * (P.tn_stack ? 2u : 1u)) / 4u;
+}
+
 // returns false for padding indices of a batched step
 template <bool DBG>
 OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
@@ -111,15 +119,22 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
 // 0.99 part added back, so a far surface is intersected from close by. A miss reports
 // distance 1, as the miss program writes it (CSGOptiX7.cu:666-670), so h.t is the
 // reference's prd distance in every case. One trace call site serves both passes.
-template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW, bool RADIAL, bool SIMPLE = false>
+// CSGL: the node and itra arrays are read from the LDS copy stage_csg made.
+template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW, bool RADIAL, bool SIMPLE = false,
+          bool CSGL = false>
 OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int* stk, int stride,
                        const BNode* top, int ntop, uint32_t& visits, uint32_t& tests, uint32_t& err,
                        uint32_t& csgb, uint32_t& ivis) {
+    Geom g = P.g;
+    if constexpr (CSGL) {
+        g.node = csg_copy(P);
+        g.itra = g.node + P.csg_node4;
+    }
     f3 o = pos;
     float t_approx = 0.f;
     bool refined = false, found;
     for (;;) {
-        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW, RADIAL, SIMPLE>(P.g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap,
+        found = trace<FP64, COUNT, LIST, TN, FLAT, SHALLOW, RADIAL, SIMPLE>(g, o, mom, tmin, P.tmax, h, stk, stride, P.stack_cap,
                                                                    top, ntop, visits, tests, err, csgb, ivis);
         if (!found) h.t = 1.f;
         if (!P.refine || refined) break;
@@ -139,7 +154,8 @@ OXD bool step_photon(const Params& P, Lane& L, int* stk, int stride, const BNode
     if (!(L.bounce < P.max_bounce && L.p.time < P.max_time)) return false;
     float tmin = (L.p.obf & P.eps0_mask) ? P.tmin0 : P.tmin;
     Hit h;
-    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0, (TM & 8) != 0, (TM & 16) != 0>(
+    bool found = trace_refined<FP64, COUNT, LIST, (TM & 1) != 0, (TM & 2) != 0, (TM & 4) != 0, (TM & 8) != 0, (TM & 16) != 0,
+                               (TM & 32) != 0>(
         P, L.p.pos, L.p.mom, tmin, h, stk, stride, top, ntop, visits, tests, err, csgb, ivis);
     if (!found) {
         if (DBG) { L.lc = 0.f; L.lf = 0.f; }   // __miss__ms sets lposcost = lposfphi = 0
@@ -203,6 +219,18 @@ OXD void stage_top(const Params& P, BNode* top, int ntop) {
     for (int i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = P.g.tlas[i];
 }
 
+// TM bit 5 (32): the node and itra arrays are copied into LDS after the top nodes and the
+// traversal stacks, and the trace reads them there. The copy's address derives from the
+// LDS array alone, so its reads compile to ds_read.
+template <int TM>
+OXD void stage_csg(const Params& P) {
+    if constexpr ((TM & 32) != 0) {
+        float4* l = const_cast<float4*>(csg_copy(P));
+        for (int i = threadIdx.x; i < P.csg_node4; i += blockDim.x) l[i] = P.g.node[i];
+        for (int i = threadIdx.x; i < P.csg_itra4; i += blockDim.x) l[P.csg_node4 + i] = P.g.itra[i];
+    }
+}
+
 OXD unsigned long long wave_sum(unsigned long long v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
@@ -232,6 +260,7 @@ __global__ void __launch_bounds__(256, OXR_PLAIN_WAVES) k_simulate(Params P) {
     BNode* top = reinterpret_cast<BNode*>(smem4);
     int* stk = reinterpret_cast<int*>(top + P.ntop) + threadIdx.x;
     stage_top(P, top, P.ntop);
+    stage_csg<TM>(P);
     __syncthreads();
     long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long steps = 0, visits = 0, tests = 0, csgb = 0, ivis = 0;
@@ -262,6 +291,7 @@ __global__ void __launch_bounds__(256, OXR_PERSIST_WAVES) k_simulate_persist(Par
     BNode* top = reinterpret_cast<BNode*>(smem4);
     int* stk = reinterpret_cast<int*>(top + P.ntop) + threadIdx.x;
     stage_top(P, top, P.ntop);
+    stage_csg<TM>(P);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     Lane L;
@@ -384,6 +414,35 @@ hipError_t launch_simulate_persist(hipStream_t s, const oxd::Params& P, dim3 g, 
     }
 #undef OXR_PERSIST
     return hipGetLastError();
+}
+
+template <int BASE>
+hipError_t launch_simulate_simple(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool persist, int tm) {
+#define OXR_S(T)                                                                                          \
+    do {                                                                                                  \
+        if (persist) hipLaunchKernelGGL((oxd::k_simulate_persist<false, false, false, BASE + T>), g, b, sm, s, P); \
+        else hipLaunchKernelGGL((oxd::k_simulate<false, false, false, false, BASE + T>), g, b, sm, s, P);  \
+    } while (0)
+    switch (tm & 15) {
+        case 0: OXR_S(0); break;   case 1: OXR_S(1); break;   case 2: OXR_S(2); break;   case 3: OXR_S(3); break;
+        case 4: OXR_S(4); break;   case 5: OXR_S(5); break;   case 6: OXR_S(6); break;   case 7: OXR_S(7); break;
+        case 8: OXR_S(8); break;   case 9: OXR_S(9); break;   case 10: OXR_S(10); break; case 11: OXR_S(11); break;
+        case 12: OXR_S(12); break; case 13: OXR_S(13); break; case 14: OXR_S(14); break; default: OXR_S(15); break;
+    }
+#undef OXR_S
+    return hipGetLastError();
+}
+
+template <int BASE>
+const void* persist_kernel_simple(int tm) {
+#define OXR_K(T) return reinterpret_cast<const void*>(&oxd::k_simulate_persist<false, false, false, BASE + T>)
+    switch (tm & 15) {
+        case 0: OXR_K(0);   case 1: OXR_K(1);   case 2: OXR_K(2);   case 3: OXR_K(3);
+        case 4: OXR_K(4);   case 5: OXR_K(5);   case 6: OXR_K(6);   case 7: OXR_K(7);
+        case 8: OXR_K(8);   case 9: OXR_K(9);   case 10: OXR_K(10); case 11: OXR_K(11);
+        case 12: OXR_K(12); case 13: OXR_K(13); case 14: OXR_K(14); default: OXR_K(15);
+    }
+#undef OXR_K
 }
 
 } // namespace oxl

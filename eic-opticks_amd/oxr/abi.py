@@ -81,7 +81,7 @@ class EngineInfo(C.Structure):
                 ("free_bytes", C.c_int64), ("total_bytes", C.c_int64),
                 ("device_name", C.c_char * 64), ("arch", C.c_char * 32),
                 ("csg_height", C.c_int32), ("csg_in_registers", C.c_int32), ("simple_leaves", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("csg_in_lds", C.c_int32)]
 
 
 class ServerConfig(C.Structure):

@@ -15,7 +15,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(PKG), 'csrc')
 BUILD = os.path.join(os.path.dirname(PKG), '_build')
 LIB = os.path.join(PKG, 'liboxr.so')
-SOURCES = ['oxr_kernels.hip', 'oxr_sim_f32.hip', 'oxr_sim_f32_simple.hip', 'oxr_sim_f32_list.hip', 'oxr_sim_f64.hip', 'oxr_sim_f64_list.hip', 'oxr_merge.hip', 'oxr_noalloc.hip', 'oxr_engine.cpp', 'oxr_bvh.cpp', 'oxr_server.cpp', 'oxr_foundry.cpp']
+SOURCES = ['oxr_kernels.hip', 'oxr_sim_f32.hip', 'oxr_sim_f32_simple.hip', 'oxr_sim_f32_simple_lds.hip', 'oxr_sim_f32_list.hip', 'oxr_sim_f64.hip', 'oxr_sim_f64_list.hip', 'oxr_merge.hip', 'oxr_noalloc.hip', 'oxr_engine.cpp', 'oxr_bvh.cpp', 'oxr_server.cpp', 'oxr_foundry.cpp']
 ARCH = os.environ.get('OXR_OFFLOAD_ARCH', 'gfx950')
 HIPCC = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
 

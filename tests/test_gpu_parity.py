@@ -486,6 +486,54 @@ def test_simple_leaf_kernels_chosen_by_the_leaves(engine_mod, oracle_lib):
         assert np.array_equal(bits(got), bits(want)), with_cone
 
 
+
+def test_csg_arrays_read_from_lds_when_they_fit(engine_mod, oracle_lib):
+    """The simple-leaf kernels copy the CSG node and transform arrays into LDS when they
+    take at most 16 KB and the copy costs no resident workgroup: 60 placed boxes (7.7 KB
+    with their transforms) are copied, 300 (38 KB) are not, and the instanced-CSG event at
+    its tuned launch is. Photons equal the oracle's with and without the copy."""
+    import json
+    from oxr import autotune
+    rng = np.random.default_rng(5)
+    for nbox, want in ((60, 1), (300, 0)):
+        f = Foundry()
+        f.add_material('Air', 1.0)
+        f.add_material('Glass', 1.5)
+        b0 = f.boundary('Air', None, None, 'Air')
+        b1 = f.boundary('Air', None, None, 'Glass')
+        f.begin_solid('r0')
+        f.add_prim(box3(3000.0), b0)
+        for x, y, z in rng.uniform(-1200.0, 1200.0, size=(nbox, 3)):
+            f.add_prim(box3(40.0).moved(translate(float(x), float(y), float(z))), b1)
+        f.end_solid()
+        f.add_instance(np.eye(4), 0)
+        g, p = f.arrays(), f.physics()
+        gs = G.torch([20000], pos=(0.0, 0.0, 0.0), wavelength=440.0, type=G.T_POINT_ISOTROPIC)
+        cfg = abi.default_config()
+        print('boxes', nbox, flush=True)
+        with engine_mod.Engine(g, p, cfg) as e:
+            i = e.info()
+            assert (i['simple_leaves'], i['csg_in_lds']) == (1, want), nbox
+            e.set_gensteps(gs)
+            e.launch(abi.OXR_FORM_PERSISTENT)
+            got = e.photons()
+        o = oracle_lib.Oracle(g, p, cfg)
+        o.set_cull(True)
+        assert np.array_equal(bits(got), bits(o.simulate(gs, seq=False)['photon'])), nbox
+    s = scene('instanced_csg')
+    cfg = abi.default_config(**s.config)
+    gs = s.gensteps(scale_photons=SCENES['instanced_csg'][1])
+    with open(autotune.TABLE) as fh:
+        tune = json.load(fh)['instanced_csg']
+    with engine_mod.Engine(s.geometry, s.physics, cfg) as e:
+        e.set_tuning(**tune)
+        assert e.info()['csg_in_lds'] == 1
+        e.set_gensteps(gs)
+        e.launch(abi.OXR_FORM_PERSISTENT)
+        got = e.photons()
+    want = run_oracle(oracle_lib, s, gs, cfg, cull=True)['photon']
+    assert np.array_equal(bits(got), bits(want))
+
 def test_max_photon_from_device_memory(engine_mod):
     """max_photon 0 (the default, as the reference's MaxSlot 0) sizes a launch slice from
     the device memory: 87% of it over the bytes per photon, rounded down to a million
