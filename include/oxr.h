@@ -375,7 +375,9 @@ int oxr_intersect_prim(oxr_engine* e, int32_t prim_index, const float* rays, int
 
 /* ---- autotuner ----------------------------------------------------------
  * Times the persistent form on the current event over a grid of oxr_tuning values
- * (repeat times each) and applies the fastest. results (max_results,OXR_AUTOTUNE_COLUMNS)
+ * (repeat times each), times the four fastest again in turn over three rounds (their
+ * rows then report the mean of the rounds, as does any row that became the fastest
+ * before it was timed so) and applies the fastest. results (max_results,OXR_AUTOTUNE_COLUMNS)
  * f64 = block_size, blocks_per_cu, lds_top_nodes, refill_batch, stack_mode, bvh_mode,
  * csg_stack, bvh_radial, photon_order, generate_inplace, kernel_ms,
  * one row per distinct effective tuning launched (see oxr_engine_set_tuning);
