@@ -66,13 +66,13 @@ bool Engine::tn_stack_for(const oxr_tuning& t, int block) const {
     return need <= share;
 }
 
-// The simple-leaf kernels read the node and itra arrays from a copy in LDS when the arrays
-// take at most 16 KB and the copy leaves as many blocks of `block` lanes resident per CU
+// The simple-leaf kernels read the node and itra arrays and the prim table from a copy in
+// LDS when they take at most 16 KB and the copy leaves as many blocks of `block` lanes resident per CU
 // as the launch has without it (smem bytes): 160 KB of LDS per CU, and at most 16 waves
 // per CU at the kernels' 4 waves per SIMD. A copy that cost a resident block would cost
 // more in latency hiding than its LDS reads save.
 bool Engine::csg_copy_for(size_t smem, int block) const {
-    const size_t bytes = size_t(num_node + num_itra) * 64;
+    const size_t bytes = size_t(num_node + num_itra) * 64 + size_t(num_prim) * 8;
     if (bytes > 16 * 1024 || smem + bytes > 64 * 1024 || block < 64) return false;
     const size_t waves = size_t(16 / (block / 64));
     auto resident = [&](size_t sm) { return std::min(waves, size_t(160 * 1024) / std::max<size_t>(sm, 1)); };
@@ -125,10 +125,11 @@ oxd::Params Engine::params(EventBuffers& b, int64_t n, int64_t slot_off, int32_t
     P.tn_stack = tn ? 1 : 0;
     P.shallow_csg = tune.csg_stack == 1 && registers_csg_ok() ? 1 : 0;
     P.simple_leaves = simple_leaves && !has_list ? 1 : 0;
-    P.csg_node4 = P.csg_itra4 = 0;
+    P.csg_node4 = P.csg_itra4 = P.csg_prim = 0;
     if (P.simple_leaves && csg_copy_for(oxl::simulate_smem(P, block), block)) {
         P.csg_node4 = num_node * 4;
         P.csg_itra4 = num_itra * 4;
+        P.csg_prim = num_prim;
     }
     P.gen_inplace = tune.generate_inplace == 1 ? 1 : 0;
     P.radial = tune.bvh_radial == 1 ? 1 : 0;

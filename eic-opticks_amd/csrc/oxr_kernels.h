@@ -62,7 +62,8 @@ struct Params {
     int radial;                // nodes also cull by their radial bounds (tuning bvh_radial)
     int simple_leaves;         // every leaf is a sphere, zsphere, cylinder or box (no list nodes)
     int csg_node4, csg_itra4;  // > 0: the simple-leaf kernels copy the node and itra arrays (this many
-                               // float4) into LDS after the traversal stacks and read them there
+    int csg_prim;              // float4) and the prim table (this many int2) into LDS after the
+                               // traversal stacks and read them there
     const unsigned* perm;      // persistent form: the order photons are claimed in, or null (index order)
     int gen_inplace;           // production kernels generate each photon where they claim it
     int refine;                // PropagateRefine: trace again from 0.99 of a far hit
@@ -105,7 +106,8 @@ hipError_t noalloc_setup(int device);
 // production kernels of geometries whose leaves are all spheres, zspheres, cylinders or
 // boxes, with single-precision roots and no list nodes: the traversal mode tm (bits 0-3)
 // plus BASE, 16 (the simple-leaf bit; oxr_sim_f32_simple.hip) or 48 (also bit 5: node and
-// itra arrays read from their LDS copy, P.csg_node4 > 0; oxr_sim_f32_simple_lds.hip)
+// itra arrays and the prim table read from their LDS copy, P.csg_node4 > 0;
+// oxr_sim_f32_simple_lds.hip)
 template <int BASE>
 hipError_t launch_simulate_simple(hipStream_t s, const oxd::Params& P, dim3 g, dim3 b, size_t sm, bool persist, int tm);
 template <int BASE>

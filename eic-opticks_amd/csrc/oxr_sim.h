@@ -60,7 +60,8 @@ OXD void point(const Params& P, Lane& L) {
     }
 }
 
-// the LDS copy of the node and itra arrays (TM bit 5): after the P.ntop staged top nodes
+// the LDS copy of the node and itra arrays and the prim table (TM bit 5): after the P.ntop
+// staged top nodes
 // and the traversal stacks (blockDim.x lanes x P.stack_cap entries of 4 B, 8 B with
 // distances), the layout simulate_smem sizes
 OXD const float4* csg_copy(const Params& P) {
@@ -119,7 +120,7 @@ OXD bool begin_photon(const Params& P, Lane& L, long long idx) {
 // 0.99 part added back, so a far surface is intersected from close by. A miss reports
 // distance 1, as the miss program writes it (CSGOptiX7.cu:666-670), so h.t is the
 // reference's prd distance in every case. One trace call site serves both passes.
-// CSGL: the node and itra arrays are read from the LDS copy stage_csg made.
+// CSGL: the node and itra arrays and the prim table are read from the LDS copy stage_csg made.
 template <bool FP64, bool COUNT, bool LIST, bool TN, bool FLAT, bool SHALLOW, bool RADIAL, bool SIMPLE = false,
           bool CSGL = false>
 OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int* stk, int stride,
@@ -129,6 +130,7 @@ OXD bool trace_refined(const Params& P, f3 pos, f3 mom, float tmin, Hit& h, int*
     if constexpr (CSGL) {
         g.node = csg_copy(P);
         g.itra = g.node + P.csg_node4;
+        g.prim = reinterpret_cast<const int2*>(g.itra + P.csg_itra4);
     }
     f3 o = pos;
     float t_approx = 0.f;
@@ -219,8 +221,8 @@ OXD void stage_top(const Params& P, BNode* top, int ntop) {
     for (int i = threadIdx.x; i < ntop; i += blockDim.x) top[i] = P.g.tlas[i];
 }
 
-// TM bit 5 (32): the node and itra arrays are copied into LDS after the top nodes and the
-// traversal stacks, and the trace reads them there. The copy's address derives from the
+// TM bit 5 (32): the node and itra arrays and the prim table are copied into LDS after the
+// top nodes and the traversal stacks, and the trace reads them there. The copy's address derives from the
 // LDS array alone, so its reads compile to ds_read.
 template <int TM>
 OXD void stage_csg(const Params& P) {
@@ -228,6 +230,8 @@ OXD void stage_csg(const Params& P) {
         float4* l = const_cast<float4*>(csg_copy(P));
         for (int i = threadIdx.x; i < P.csg_node4; i += blockDim.x) l[i] = P.g.node[i];
         for (int i = threadIdx.x; i < P.csg_itra4; i += blockDim.x) l[P.csg_node4 + i] = P.g.itra[i];
+        int2* lp = reinterpret_cast<int2*>(l + P.csg_node4 + P.csg_itra4);
+        for (int i = threadIdx.x; i < P.csg_prim; i += blockDim.x) lp[i] = P.g.prim[i];
     }
 }
 

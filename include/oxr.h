@@ -225,9 +225,10 @@ typedef struct oxr_engine_info {
                                  prim holds a list node, so production launches with single-
                                  precision roots run kernels compiled with those leaves only */
     int32_t csg_in_lds;       /* 1: those kernels, launched in the persistent form with the
-                                 current tuning, copy the CSG node and transform arrays into
-                                 LDS and read them there (the arrays take at most 16 KB and
-                                 the copy leaves as many workgroups resident per CU)        */
+                                 current tuning, copy the CSG node and transform arrays and
+                                 the prim table into LDS and read them there (they take at
+                                 most 16 KB and the copy leaves as many workgroups resident
+                                 per CU)                                                    */
 } oxr_engine_info;
 
 typedef struct oxr_engine oxr_engine;

@@ -488,10 +488,11 @@ def test_simple_leaf_kernels_chosen_by_the_leaves(engine_mod, oracle_lib):
 
 
 def test_csg_arrays_read_from_lds_when_they_fit(engine_mod, oracle_lib):
-    """The simple-leaf kernels copy the CSG node and transform arrays into LDS when they
-    take at most 16 KB and the copy costs no resident workgroup: 60 placed boxes (7.7 KB
-    with their transforms) are copied, 300 (38 KB) are not, and the instanced-CSG event at
-    its tuned launch is. Photons equal the oracle's with and without the copy."""
+    """The simple-leaf kernels copy the CSG node and transform arrays and the prim table
+    into LDS when they take at most 16 KB and the copy costs no resident workgroup: 60
+    placed boxes (8.2 KB with their transforms and prim entries) are copied, 300 (41 KB) are
+    not, and the instanced-CSG event at its tuned launch is. Photons equal the oracle's
+    with and without the copy."""
     import json
     from oxr import autotune
     rng = np.random.default_rng(5)
