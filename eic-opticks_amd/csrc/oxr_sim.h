@@ -13,7 +13,8 @@
 // Minimum waves per SIMD the simulate kernels are compiled for (VGPR budget 512/N per
 // lane). The latency-bound traversal wants occupancy more than registers: measured on
 // MI355X (crystal-calorimeter event), 4 waves beat 3 and 5 for both forms; with the
-// simple-leaf kernels 5 waves lost on every event (instanced CSG 11.4 against 8.8 ms).
+// simple-leaf kernels 5 waves lost on every event (instanced CSG 11.4 against 8.8 ms), and
+// with the CSG arrays in LDS 3 waves lost too (instanced CSG 9.0 against 7.9 ms).
 #ifndef OXR_PERSIST_WAVES
 #define OXR_PERSIST_WAVES 4
 #endif
