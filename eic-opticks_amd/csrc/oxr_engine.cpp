@@ -769,6 +769,7 @@ int oxr_engine_get_config(oxr_engine* eh, oxr_config* c) {
 int oxr_engine_query(oxr_engine* eh, oxr_engine_info* info) {
     Engine* e = reinterpret_cast<Engine*>(eh);
     if (!e || !info) return fail(OXR_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(e->mu);   // tuning and event buffers are read below
     std::memset(info, 0, sizeof(*info));
     info->num_solid = e->num_solid; info->num_prim = e->num_prim; info->num_node = e->num_node; info->num_inst = e->num_inst;
     info->tlas_nodes = e->tlas_nodes; info->blas_nodes = e->blas_nodes;
